@@ -1,0 +1,149 @@
+"""Benchmark: zone-updates/s per RK stage of the fused isothermal stage on one MI355X.
+
+Workload (one "step" = one RK3 step = 3 fused stage launches over the grid):
+    4096 x 4096, nvar = 3, PPM4 + Roe, RK3, cs = 1, CFL 0.4, dfloor 1e-10, no viscosity,
+    drag or forcing; smooth-wave initial state rho = 1 + 0.2 sin(2 pi x) cos(2 pi y),
+    vx = 0.1 sin(2 pi y), vy = 0.1 cos(2 pi x); FP32 storage and arithmetic (FP64 time).
+The same steps are then timed in FP64 for the FP32/FP64 ratio.  The state is resident in
+HBM before the timed region; the time loop runs on the device (evolve_isothermal with
+nlim = steps), bracketed by torch.cuda.synchronize() on both sides.
+
+Prints ONE JSON line.  `roofline` describes the fused stage kernel: algorithmic bytes per
+launch (stage 1 reads and writes U; stages 2-3 also read the saved register: on average
+(8/3)*nvar*w bytes per zone) divided by its average launch time measured with HIP events
+on the launch stream.  `cpu_baseline` times the NumPy oracle on one host core.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "zone-updates/s per RK stage, FP32 vs FP64 at 4096²; p50/p99 ensemble latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096, help="grid size (n x n)")
+    ap.add_argument("--dtype", choices=("f32", "f64"), default="f32")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=2048, help="CPU baseline sample grid size")
+    ap.add_argument("--block-x", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    return ap.parse_args()
+
+
+def gpu_leg(jx, problems, n, dtype, steps, warmup, block_x, rows):
+    import torch
+    from jathenax_amd.device import context_for
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    U0 = torch.tensor(problems.smooth_wave(n, n), dtype=tdt, device="cuda")
+    ctx = context_for(U0)
+    ctx.set_tiles(block_x, rows)
+    args = (1.0 / n, 1.0 / n, 1.0, 1e-10, "rk3", 0.4, 1.0e9)
+    if warmup:
+        jx.evolve_isothermal(U0, *args, warmup)
+    torch.cuda.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    U, t, dt, cyc = jx.evolve_isothermal(U0, *args, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launches, kernel_ms = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    assert cyc == steps, (cyc, steps)
+    finite = bool(torch.isfinite(U).all())
+    del U
+    return dict(wall=wall, launches=launches, kernel_ms=kernel_ms, finite=finite)
+
+
+def cpu_leg(problems, n):
+    sys.path.insert(0, str(ROOT))
+    from oracle import jathenax_np as ox
+    U = problems.smooth_wave(n, n)
+    t0 = time.perf_counter()
+    ox.isothermal_step(U, 1e-4, 1.0 / n, 1.0 / n, 1.0, 1e-10, "rk3")
+    sec = time.perf_counter() - t0
+    return n * n * 3 / sec, sec
+
+
+def read_traffic(n, dtype):
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary, if present."""
+    f = ROOT / "profiles" / "stage_traffic.json"
+    if not f.exists():
+        return None
+    try:
+        data = json.loads(f.read_text())
+        return data.get(f"{n}x{n}_{dtype}")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the single-grid "
+              "path would shard by y-slabs with halo exchange, which this build does not run)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import jathenax_amd as jx
+    from jathenax_amd import problems
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    n, K = a.n, a.steps
+    nvar, stages = 3, 3
+    print(f"# bench: {n}x{n} {a.dtype} steps={K} warmup={a.warmup}", file=sys.stderr, flush=True)
+    main_leg = gpu_leg(jx, problems, n, a.dtype, K, a.warmup, a.block_x, a.rows)
+    zu = n * n * stages * K / main_leg["wall"]
+    w = 4 if a.dtype == "f32" else 8
+    bytes_per_launch = (8.0 / 3.0) * nvar * w * n * n
+    avg_launch_s = main_leg["kernel_ms"] / 1e3 / max(1, main_leg["launches"])
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC, "value": zu, "unit": "zone-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": a.warmup, "ms_per_step": main_leg["wall"] / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": a.dtype, "data": "synthetic",
+        "config": {"workload": f"{n}x{n} smooth wave, PPM4+Roe, RK3, nvar=3, CFL 0.4, "
+                               f"device-resident time loop", "nx": n, "ny": n,
+                   "reconstruct": "ppm4", "rsolver": "roe", "integrator": "rk3",
+                   "stages_per_step": stages},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(n, a.dtype),
+                     "kernel": "jx::stage_kernel (fused RK stage)",
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": avg_launch_s * 1e3, "launches_timed": main_leg["launches"]},
+        "finite": main_leg["finite"],
+    }
+    if not a.no_fp64 and a.dtype == "f32":
+        print("# bench: fp64 leg", file=sys.stderr, flush=True)
+        f64 = gpu_leg(jx, problems, n, "f64", K, a.warmup, a.block_x, a.rows)
+        zu64 = n * n * stages * K / f64["wall"]
+        out["fp64"] = {"value": zu64, "ms_per_step": f64["wall"] / K * 1e3,
+                       "avg_launch_ms": f64["kernel_ms"] / max(1, f64["launches"]),
+                       "fp32_over_fp64": zu / zu64}
+    if not a.no_cpu_baseline:
+        print("# bench: cpu baseline", file=sys.stderr, flush=True)
+        rate, sec = cpu_leg(problems, a.cpu_n)
+        out["cpu_baseline"] = {
+            "value": rate, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": f"NumPy float64 oracle, one RK3 step of the same workload on a "
+                      f"{a.cpu_n}x{a.cpu_n} grid ({sec:.1f} s; host has {os.cpu_count()} cores, "
+                      f"NumPy elementwise ops use one)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

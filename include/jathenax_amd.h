@@ -1,0 +1,141 @@
+/*
+ * jathenax_amd: MI355X (gfx950) per-RK-stage update of JathenAX's periodic isothermal
+ * hydro solver, behind a C ABI.
+ *
+ * Conventions
+ *  - Arrays are C-contiguous device pointers in the reference layout: the state is
+ *    (ny, nx, nvar) with components (rho, rho*vx, rho*vy, rho*s...), forces (ny, nx, 2).
+ *  - dtype 0 = float64, 1 = float32 (FP32 storage and arithmetic; time, box sums and the
+ *    kick normalisation stay float64).
+ *  - Every call is ordered on the context's HIP stream and never frees caller memory.
+ *    Calls that return host scalars synchronise the stream.  A context is not
+ *    thread-safe.
+ *  - Return value: JX_OK, or an error code whose message is jx_last_error(ctx).
+ *    JX_EINVAL maps to the reference's ValueError, JX_EKEY to its KeyError (unknown
+ *    integrator / reconstruction / Riemann solver), JX_EHIP to a runtime failure.
+ *  - Reference citations are to the JathenAX sources (the jathenax Python package).
+ */
+#ifndef JATHENAX_AMD_H
+#define JATHENAX_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JX_OK 0
+#define JX_EINVAL 1
+#define JX_EKEY 2
+#define JX_EHIP 3
+
+/* Reconstruction ids, in the order of RECONSTRUCTIONS (jathenax/fluxes.py:120-123). */
+#define JX_RECON_DC 0
+#define JX_RECON_PLM 1
+#define JX_RECON_PPM4 2
+#define JX_RECON_PPMX 3
+#define JX_RECON_WENOZ 4
+/* Riemann solver ids, in the order of RIEMANN_SOLVERS (jathenax/fluxes.py:204). */
+#define JX_RS_LLF 0
+#define JX_RS_HLLE 1
+#define JX_RS_ROE 2
+
+typedef struct jx_ctx jx_ctx;
+
+/* Run parameters: the static and scalar arguments of isothermal_step
+ * (jathenax/hydro.py:92-95) and evolve_isothermal (jathenax/run.py:24-27). */
+typedef struct jx_run_params {
+  int ny, nx, nvar;          /* state shape; nvar in {3, 4, 5} */
+  int dtype;                 /* 0 float64, 1 float32 */
+  int reconstruct, rsolver;  /* JX_RECON_*, JX_RS_* */
+  int integrator;            /* RK_STAGES key: 2 = rk2, 3 = rk3, 4 = rk4 */
+  int nlim;                  /* cycle limit (evolve only) */
+  double dx, dy, cs, dfloor, cfl, tlim;
+  double viscosity, smagorinsky_cs, drag_rate;
+  double dt_update, tdriv_start, dedt;   /* forcing schedule (evolve only) */
+} jx_run_params;
+
+/* Forcing tables (jathenax/run.py:148-155): the OU coefficient table and the phase
+ * factors of forcing_basis, all device arrays in the state dtype. */
+typedef struct jx_forcing {
+  const void* coefficients;  /* (n_updates, n_modes, 2, 2) */
+  const void* cos_x;         /* (n_modes, nx) */
+  const void* sin_x;
+  const void* cos_y;         /* (n_modes, ny) */
+  const void* sin_y;
+  int n_updates, n_modes;
+} jx_forcing;
+
+/* Context: one device, one stream (0 = the device's null stream), owned scratch. */
+int jx_ctx_create(jx_ctx** out, int device, void* stream);
+void jx_ctx_destroy(jx_ctx* ctx);
+const char* jx_last_error(const jx_ctx* ctx);
+/* Tile shape of the fused stage kernel: block_x threads (64..256, multiple of 64) per
+ * strip of block_x-6 columns, `rows` rows per block; 0 selects the default. */
+int jx_set_tiles(jx_ctx* ctx, int block_x, int rows);
+/* Measurement hooks (no reference counterpart): when on, every fused stage launch is
+ * bracketed by HIP events on the context stream; jx_kernel_timing synchronises and
+ * returns the number of timed launches and their summed duration, then resets. */
+int jx_set_kernel_timing(jx_ctx* ctx, int on);
+int jx_kernel_timing(jx_ctx* ctx, int* launches, double* total_ms);
+
+/* One step, all RK stages (replaces isothermal_step, jathenax/hydro.py:92-146).
+ * U_in is not modified; the floored result goes to U_out (distinct from U_in).
+ * raw_force may be NULL; otherwise the final stage is kicked with power `dedt`. */
+int jx_isothermal_step(jx_ctx* ctx, const jx_run_params* p, const void* U_in, void* U_out,
+                       double dt, const void* raw_force, double dedt);
+
+/* Timestep limit from primitives W (replaces isothermal_timestep, hydro.py:63-89). */
+int jx_isothermal_timestep(jx_ctx* ctx, const jx_run_params* p, const void* W,
+                           double previous_dt, double time, double tlim, double* dt_out);
+
+/* Device-resident time loop (replaces evolve_isothermal, jathenax/run.py:24-69).
+ * forcing may be NULL.  Returns (time, last dt, cycle) like the reference. */
+int jx_evolve_isothermal(jx_ctx* ctx, const jx_run_params* p, const void* U_in, void* U_out,
+                         const jx_forcing* forcing, double* time_out, double* dt_out,
+                         int* cycle_out);
+
+/* recover_isothermal (hydro.py:54-60): floored U and primitives W; either may be NULL. */
+int jx_recover_isothermal(jx_ctx* ctx, int dtype, long long ncells, int nvar, const void* U,
+                          double dfloor, void* U_out, void* W_out);
+
+/* rk_stage (hydro.py:39-51) on n values. U_saved_out may be NULL when has_delta == 0. */
+int jx_rk_stage(jx_ctx* ctx, int dtype, long long n, const void* U, const void* U_saved,
+                const void* div_flux, double dt, double gam0, double gam1, double beta,
+                double delta, int has_delta, void* U_next, void* U_saved_out);
+
+/* flux_divergence (fluxes.py:283-287) of (ny, nx, nvar) face fluxes. */
+int jx_flux_divergence(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* flux_x,
+                       const void* flux_y, double dx, double dy, void* out);
+
+/* stress_face_fluxes (fluxes.py:269-280): (ny, nx, 3) stress to two (ny, nx, 2) arrays. */
+int jx_stress_face_fluxes(jx_ctx* ctx, int dtype, int ny, int nx, const void* tau,
+                          void* flux_x, void* flux_y);
+
+/* render_force (forcing.py:111-122) of one OU row `amplitudes` (n_modes, 2, 2). */
+int jx_render_force(jx_ctx* ctx, int dtype, int ny, int nx, int n_modes,
+                    const void* amplitudes, const void* cos_x, const void* sin_x,
+                    const void* cos_y, const void* sin_y, void* raw_out);
+
+/* normalize_force (forcing.py:137-149): acceleration (ny, nx, 2). */
+int jx_normalize_force(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* U,
+                       const void* raw_force, double dt, double dedt, void* acc_out);
+
+/* Float64 box sums of each component (run.py:186-187 before the dx*dy factor). */
+int jx_conserved_sums(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* U,
+                      double* sums_out);
+
+/* Seed path on the host, float64 (forcing.py:11-100). */
+int jx_ran_st(long long seed, long long count, double* out);            /* forcing.py:11 */
+int jx_ran_gaussian_st(long long seed, long long count, double* out);   /* forcing.py:35 */
+/* cartesian_modes (forcing.py:48-60): writes up to `cap` (kx, ky) pairs, returns count
+ * (or -JX_EINVAL). */
+int jx_cartesian_modes(int nlow, int nhigh, int kx_min, int kx_max, int ky_min, int ky_max,
+                       int* modes_out, int cap);
+/* modal_coefficients (forcing.py:63-100): out is (updates, n_modes, 2, 2) float64. */
+int jx_modal_coefficients(const int* modes, int n_modes, double lx, double ly, int nlow,
+                          int nhigh, double npeak, double sol_fraction, long long rseed,
+                          double tcorr, double dt_update, int updates, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JATHENAX_AMD_H */

@@ -1,0 +1,91 @@
+"""Build libjathenax_amd.so in-tree: hipcc for gfx950 device code, g++ for the seed path.
+
+Usage: python -m jathenax_amd.build [-j JOBS] [-v]
+
+FP64 translation units are compiled with -ffp-contract=off so that every multiply and
+add rounds as in the reference's IEEE float64 expressions; the FP32 stage units allow
+contraction.  All HIP code is built with packed FP32 instructions disabled.
+"""
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OBJ = PKG / "_build"
+LIB = PKG / "libjathenax_amd.so"
+ARCH = os.environ.get("JX_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _hipcc():
+    h = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(h).exists():
+        raise RuntimeError("hipcc not found: the HIP toolchain is required to build")
+    return h
+
+
+def _jobs(n=None):
+    if n:
+        return n
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def compile_commands():
+    cmds = []
+    for src in sorted(CSRC.glob("*.hip")):
+        contract = "fast" if src.name.startswith("stage_f32_") else "off"
+        obj = OBJ / (src.stem + ".o")
+        cmds.append((src, obj, [_hipcc(), *HIP_FLAGS, f"-ffp-contract={contract}", "-c",
+                                str(src), "-o", str(obj)]))
+    for src in sorted(CSRC.glob("*.cpp")):
+        obj = OBJ / (src.stem + ".o")
+        cmds.append((src, obj, ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off",
+                                "-c", str(src), "-o", str(obj)]))
+    return cmds
+
+
+def build(jobs=None, verbose=False):
+    OBJ.mkdir(exist_ok=True)
+    cmds = compile_commands()
+
+    def run(item):
+        src, obj, cmd = item
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"compiling {src.name} failed:\n{res.stdout}\n{res.stderr}")
+        return obj
+
+    with ThreadPoolExecutor(_jobs(jobs)) as pool:
+        objs = list(pool.map(run, cmds))
+    tmp = LIB.with_suffix(".so.tmp")
+    link = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+            *map(str, objs)]
+    if verbose:
+        print(" ".join(link), flush=True)
+    res = subprocess.run(link, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"linking failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", "--jobs", type=int)
+    ap.add_argument("-v", "--verbose", action="store_true")
+    args = ap.parse_args()
+    print(build(args.jobs, args.verbose))
+    sys.exit(0)

@@ -1,0 +1,507 @@
+// Auxiliary gfx950 kernels around the fused stage: timestep reduction, the device-side
+// loop control, forcing render / normalisation / kick, and the pointwise operators the
+// reference exports on their own.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jx_ops.hpp"
+#include "jx_stage.hpp"
+
+namespace jx {
+
+constexpr int kAuxBlock = 256;
+
+// ---------------------------------------------------------------------------------------
+// Timestep reduction (hydro.py:63-80): min over cells of dx/(|vx|+cs), dy/(|vy|+cs) and,
+// with Smagorinsky, max over all faces of nu_SGS.  `primitive` = the input already holds
+// W (isothermal_timestep's argument); otherwise it holds U and is recovered on read.
+// ---------------------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ void load_prim(const T* q, int nv, bool primitive, T dfloor, T& r,
+                                          T& vx, T& vy) {
+  (void)nv;
+  if (primitive) {
+    r = q[0];
+    vx = q[1];
+    vy = q[2];
+  } else {
+    r = (q[0] < dfloor) ? dfloor : q[0];
+    const T inv = dvd(T(1), r);
+    vx = inv * q[1];
+    vy = inv * q[2];
+  }
+}
+
+template <typename T> struct TimestepArgs {
+  const T* q;
+  int nx, ny, nv, primitive, smag_on;
+  T dx, dy, cs, dfloor, four_dx, four_dy, smag_c;
+  const Ctrl* ctrl;               // nullptr: always run
+  unsigned long long* dtkey;      // min slot
+  unsigned long long* nukey;      // max slot
+};
+
+__device__ __forceinline__ unsigned long long max_key_of(double x) {
+  if (x != x) return ~0ull;
+  return static_cast<unsigned long long>(__double_as_longlong(x));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) {
+  if (a.ctrl && a.ctrl->done) return;
+  __shared__ unsigned long long sred[2 * (kAuxBlock / 64)];
+  const int nx = a.nx, ny = a.ny, nv = a.nv;
+  const size_t n = (size_t)nx * ny;
+  unsigned long long kmin = ~0ull, kmax = 0ull;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(c / nx), i = (int)(c % nx);
+    T r, vx, vy;
+    load_prim(a.q + c * nv, nv, a.primitive != 0, a.dfloor, r, vx, vy);
+    const unsigned long long kx = key_of((double)dvd(a.dx, fabs(vx) + a.cs));
+    const unsigned long long ky = key_of((double)dvd(a.dy, fabs(vy) + a.cs));
+    kmin = kx < kmin ? kx : kmin;
+    kmin = ky < kmin ? ky : kmin;
+    if (a.smag_on) {
+      const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
+      const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+      auto at = [&](int jj, int ii, T& u_, T& v_) {
+        T rr;
+        load_prim(a.q + ((size_t)jj * nx + ii) * nv, nv, a.primitive != 0, a.dfloor, rr, u_, v_);
+      };
+      T u_w, v_w, u_n, v_n, u_nw, v_nw, u_s, v_s, u_sw, v_sw, u_e, v_e, u_se, v_se;
+      at(j, im, u_w, v_w);
+      at(jp, i, u_n, v_n);
+      at(jp, im, u_nw, v_nw);
+      at(jm, i, u_s, v_s);
+      at(jm, im, u_sw, v_sw);
+      at(j, ip, u_e, v_e);
+      at(jm, ip, u_se, v_se);
+      // low x face (fluxes.py:227-238, axis=1)
+      {
+        const T ux = dvd(vx - u_w, a.dx);
+        const T vxn = dvd(vy - v_w, a.dx);
+        const T uy = dvd((u_n + u_nw) - (u_s + u_sw), a.four_dy);
+        const T vyt = dvd((v_n + v_nw) - (v_s + v_sw), a.four_dy);
+        const T nu = smag_nu(ux, vyt, vxn + uy, a.smag_c);
+        const unsigned long long k = max_key_of((double)nu);
+        kmax = k > kmax ? k : kmax;
+      }
+      // low y face (axis=0)
+      {
+        const T uy = dvd(vx - u_s, a.dy);
+        const T vyn = dvd(vy - v_s, a.dy);
+        const T uxt = dvd((u_e + u_se) - (u_w + u_sw), a.four_dx);
+        const T vxt = dvd((v_e + v_se) - (v_w + v_sw), a.four_dx);
+        const T nu = smag_nu(uxt, vyn, vxt + uy, a.smag_c);
+        const unsigned long long k = max_key_of((double)nu);
+        kmax = k > kmax ? k : kmax;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(kmin, off);
+    kmin = o < kmin ? o : kmin;
+    const unsigned long long p = __shfl_xor(kmax, off);
+    kmax = p > kmax ? p : kmax;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if (lane == 0) {
+    sred[wave] = kmin;
+    sred[nw + wave] = kmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = sred[0], x = sred[nw];
+    for (int w = 1; w < nw; ++w) {
+      m = sred[w] < m ? sred[w] : m;
+      x = sred[nw + w] > x ? sred[nw + w] : x;
+    }
+    atomicMin(a.dtkey, m);
+    if (a.smag_on) atomicMax(a.nukey, x);
+  }
+}
+
+__host__ __device__ inline double value_of_max_key(unsigned long long k) {
+  if (k == ~0ull) return __builtin_nan("");
+  union { unsigned long long u; double d; } c;
+  c.u = k;
+  return c.d;
+}
+
+// NaN-propagating minimum (jnp.minimum).
+__host__ __device__ inline double nan_min(double a, double b) {
+  if (a != a || b != b) return __builtin_nan("");
+  return a < b ? a : b;
+}
+
+// dt from the reduced limits (hydro.py:68-89), with previous dt, time and tlim.
+__host__ __device__ inline double finish_timestep(double cfl_limit, double nu_max, int smag_on,
+                                                  double viscosity, double drag_rate, double h,
+                                                  double cfl, double previous_dt, double time,
+                                                  double tlim) {
+  double lim = cfl_limit;
+  if (smag_on) {
+    const double rate = (viscosity + 2.0 * nu_max) / (h * h);
+    lim = nan_min(lim, 0.1875 / rate);
+  } else if (viscosity > 0.0) {
+    lim = nan_min(lim, 0.25 * (h * h) / viscosity);
+  }
+  if (drag_rate > 0.0) lim = nan_min(lim, 1.0 / drag_rate);
+  double dt = nan_min(2.0 * previous_dt, cfl * lim);
+  if ((time < tlim) && (time + dt > tlim)) dt = tlim - time;
+  return dt;
+}
+
+struct LoopArgs {
+  double tlim, cfl, viscosity, drag_rate, h, dt_update, tdriv_start, dedt;
+  int nlim, smag_on, has_forcing, n_updates;
+};
+
+__global__ void init_ctrl_kernel(Ctrl* c) {
+  c->time = 0.0;
+  c->dt = 0.0;
+  c->prev_dt = __builtin_inf();
+  c->cycle = 0;
+  c->done = 0;
+  c->pending = 0;
+  c->update = -1;
+  c->render = 0;
+  c->power = 0.0;
+  c->dtkey = ~0ull;
+  c->nukey = 0ull;
+  c->mu[0] = c->mu[1] = 0.0;
+  c->scale = 0.0;
+  c->sum_rho = 0.0;
+}
+
+// One thread: commit the previous step, test the loop condition (run.py:36-38), pick dt
+// (run.py:42-46) and the OU row due at the cycle-start time (forcing.py:125-134).
+__global__ void begin_step_kernel(Ctrl* c, LoopArgs p) {
+  if (c->done) return;
+  if (c->pending) {
+    c->time = c->time + c->dt;
+    c->prev_dt = c->dt;
+    c->cycle += 1;
+    c->pending = 0;
+  }
+  if (!((c->time < p.tlim) && (c->cycle < p.nlim))) {
+    c->done = 1;
+    return;
+  }
+  const double dt = finish_timestep(value_of_key(c->dtkey), value_of_max_key(c->nukey),
+                                    p.smag_on, p.viscosity, p.drag_rate, p.h, p.cfl, c->prev_dt,
+                                    c->time, p.tlim);
+  c->dt = dt;
+  c->pending = 1;
+  c->dtkey = ~0ull;
+  c->nukey = 0ull;
+  c->render = 0;
+  if (p.has_forcing) {
+    int req = -1;
+    if (c->time >= p.tdriv_start) req = (int)floor((c->time - p.tdriv_start) / p.dt_update);
+    if (req > c->update) {
+      c->update = req;
+      c->render = 1;
+    }
+    c->power = (c->update >= 0) ? p.dedt : 0.0;
+  }
+}
+
+// Standalone step: dt (and the kick power) come from the caller.
+__global__ void set_step_kernel(Ctrl* c, double dt, double power) {
+  c->dt = dt;
+  c->done = 0;
+  c->power = power;
+}
+
+// ---------------------------------------------------------------------------------------
+// Forcing: render of the raw acceleration (forcing.py:111-122), normalisation
+// (forcing.py:137-149) and the kick (hydro.py:127-132).
+// ---------------------------------------------------------------------------------------
+
+template <typename T> struct RenderArgs {
+  T* raw;
+  const T* table;    // (n_updates, M, 2, 2) OU coefficients in the state dtype
+  const T* cx; const T* sx; const T* cy; const T* sy;   // (M, nx), (M, ny)
+  int nx, ny, n_modes, n_updates;
+  const Ctrl* ctrl;  // nullptr: render row `row` unconditionally
+  int row;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) render_force_kernel(RenderArgs<T> a) {
+  int row = a.row;
+  if (a.ctrl) {
+    if (a.ctrl->done || !a.ctrl->render) return;
+    row = a.ctrl->update;
+  }
+  const size_t n = (size_t)a.nx * a.ny;
+  const bool fill = (row < 0) || (row >= a.n_updates);
+  const T* amp = a.table + (size_t)(fill ? 0 : row) * a.n_modes * 4;
+  const T nan = T(__builtin_nan(""));
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(c / a.nx), i = (int)(c % a.nx);
+    T f0 = T(0), f1 = T(0);
+    for (int m = 0; m < a.n_modes; ++m) {
+      const T cxv = a.cx[(size_t)m * a.nx + i], sxv = a.sx[(size_t)m * a.nx + i];
+      const T cyv = a.cy[(size_t)m * a.ny + j], syv = a.sy[(size_t)m * a.ny + j];
+      const T re = cxv * cyv - sxv * syv;
+      const T im = cxv * syv + sxv * cyv;
+      const T* am = amp + (size_t)m * 4;   // [dir][re/im]
+      const T a00 = fill ? nan : am[0], a01 = fill ? nan : am[1];
+      const T a10 = fill ? nan : am[2], a11 = fill ? nan : am[3];
+      f0 = f0 + (re * a00 - im * a01);
+      f1 = f1 + (re * a10 - im * a11);
+    }
+    a.raw[c * 2 + 0] = f0;
+    a.raw[c * 2 + 1] = f1;
+  }
+}
+
+template <typename T> struct KickArgs {
+  T* u;                 // post-drag state (ny, nx, nv), updated in place by the kick
+  const T* raw;
+  int nx, ny, nv;
+  T dfloor, dx, dy, cs;
+  const Ctrl* ctrl;
+  double* partial;      // per-block partial sums
+  unsigned long long* dtkey;   // CFL slot for the next timestep (nullptr: none)
+};
+
+// Partial sums of rho, rho*r over a fixed cell range per block (standalone normalise
+// entry; the fused path gets these from the final stage kernel).
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) kick_first_moments_kernel(KickArgs<T> a) {
+  if (a.ctrl && a.ctrl->done) return;
+  __shared__ double sred[3 * (kAuxBlock / 64)];
+  const size_t n = (size_t)a.nx * a.ny;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const T rho = a.u[c * a.nv];
+    s0 += (double)rho;
+    s1 += (double)(rho * a.raw[c * 2]);
+    s2 += (double)(rho * a.raw[c * 2 + 1]);
+  }
+  block_reduce_store<T>(s0, s1, s2, sred, a.partial + 3 * (size_t)blockIdx.x);
+}
+
+// Sum per-block partials in a fixed order.  mode 0: mean force mu = sum(rho r)/sum(rho)
+// (forcing.py:141).  mode 1: m0, m1 and the kick scale (forcing.py:144-148).
+__global__ void reduce_kick_kernel(Ctrl* c, const double* partial, int nblocks, int mode,
+                                   double ncells) {
+  if (c->done) return;
+  __shared__ double s[3][256];
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    a0 += partial[3 * b + 0];
+    a1 += partial[3 * b + 1];
+    a2 += partial[3 * b + 2];
+  }
+  s[0][threadIdx.x] = a0;
+  s[1][threadIdx.x] = a1;
+  s[2][threadIdx.x] = a2;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + w];
+      s[1][threadIdx.x] += s[1][threadIdx.x + w];
+      s[2][threadIdx.x] += s[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mode == 0) {
+      c->sum_rho = s[0][0];
+      c->mu[0] = s[1][0] / s[0][0];
+      c->mu[1] = s[2][0] / s[0][0];
+    } else {
+      const double m0 = s[0][0] / ncells, m1 = s[1][0] / ncells;
+      const double dedt = c->power;
+      const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
+      double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
+      if (!(m0 > 1e-30)) scale = __builtin_nan("");
+      c->scale = scale;
+    }
+  }
+}
+
+// Partial sums of 0.5 rho |f|^2 dt and m.f with f = r - mu (forcing.py:142-145).
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) kick_second_moments_kernel(KickArgs<T> a) {
+  if (a.ctrl->done) return;
+  __shared__ double sred[3 * (kAuxBlock / 64)];
+  const T mu0 = T(a.ctrl->mu[0]), mu1 = T(a.ctrl->mu[1]);
+  const T dt = T(a.ctrl->dt);
+  const size_t n = (size_t)a.nx * a.ny;
+  double s0 = 0, s1 = 0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const T* q = a.u + c * a.nv;
+    const T f0 = a.raw[c * 2] - mu0, f1 = a.raw[c * 2 + 1] - mu1;
+    const T fsq = f0 * f0 + f1 * f1;
+    s0 += (double)(((T(0.5) * q[0]) * fsq) * dt);
+    s1 += (double)(q[1] * f0 + q[2] * f1);
+  }
+  block_reduce_store<T>(s0, s1, 0.0, sred, a.partial + 3 * (size_t)blockIdx.x);
+}
+
+// Apply the kick, floor the density and feed the next timestep's CFL reduction.
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) kick_apply_kernel(KickArgs<T> a) {
+  if (a.ctrl->done) return;
+  __shared__ unsigned long long sred[kAuxBlock / 64];
+  const T mu0 = T(a.ctrl->mu[0]), mu1 = T(a.ctrl->mu[1]);
+  const T dt = T(a.ctrl->dt);
+  const bool off = a.ctrl->power == 0.0;
+  const T scale = T(a.ctrl->scale);
+  const size_t n = (size_t)a.nx * a.ny;
+  unsigned long long kmin = ~0ull;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    T* q = a.u + c * a.nv;
+    const T f0 = a.raw[c * 2] - mu0, f1 = a.raw[c * 2 + 1] - mu1;
+    const T acc0 = off ? T(0) : scale * f0;
+    const T acc1 = off ? T(0) : scale * f1;
+    const T rho = q[0];
+    const T m0 = q[1] + rho * (acc0 * dt);
+    const T m1 = q[2] + rho * (acc1 * dt);
+    const T rf = (rho < a.dfloor) ? a.dfloor : rho;
+    q[0] = rf;
+    q[1] = m0;
+    q[2] = m1;
+    if (a.dtkey) {
+      const T inv = dvd(T(1), rf);
+      const unsigned long long kx = key_of((double)dvd(a.dx, fabs(inv * m0) + a.cs));
+      const unsigned long long ky = key_of((double)dvd(a.dy, fabs(inv * m1) + a.cs));
+      kmin = kx < kmin ? kx : kmin;
+      kmin = ky < kmin ? ky : kmin;
+    }
+  }
+  if (a.dtkey) block_min_key(kmin, sred, a.dtkey);
+}
+
+// Standalone normalize_force output: the acceleration field (forcing.py:149).
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) kick_acceleration_kernel(const Ctrl* c,
+                                                                      const T* raw, T* out,
+                                                                      size_t n) {
+  const T mu0 = T(c->mu[0]), mu1 = T(c->mu[1]);
+  const bool off = c->power == 0.0;
+  const T scale = T(c->scale);
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    out[2 * k] = off ? T(0) : scale * (raw[2 * k] - mu0);
+    out[2 * k + 1] = off ? T(0) : scale * (raw[2 * k + 1] - mu1);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Pointwise operators exported by the reference (hydro.py:39-60, fluxes.py:269-287) and
+// the float64 box sums of run.py:186-187.
+// ---------------------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) recover_kernel(const T* u, T* uf, T* w, size_t n,
+                                                            int nv, T dfloor) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const T* q = u + c * nv;
+    const T rho = (q[0] < dfloor) ? dfloor : q[0];
+    const T inv = dvd(T(1), rho);
+    if (uf) {
+      uf[c * nv] = rho;
+      for (int v = 1; v < nv; ++v) uf[c * nv + v] = q[v];
+    }
+    if (w) {
+      w[c * nv] = rho;
+      for (int v = 1; v < nv; ++v) w[c * nv + v] = inv * q[v];
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) rk_stage_kernel(const T* u, const T* us,
+                                                             const T* div, T* out, T* us_out,
+                                                             size_t n, double dt, StageCoef k) {
+  const T sdt = T(k.beta * dt);
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    T s = us[c];
+    if (k.has_delta) s = s + T(k.delta) * u[c];
+    if (k.blended)
+      out[c] = s + (T(k.gam0) * (u[c] - s) - sdt * div[c]);
+    else
+      out[c] = (T(k.gam0) * u[c] + T(k.gam1) * s) - sdt * div[c];
+    if (us_out) us_out[c] = s;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) flux_divergence_kernel(const T* fx, const T* fy,
+                                                                    T* out, int nx, int ny,
+                                                                    int nv, T dx, T dy) {
+  const size_t n = (size_t)nx * ny * nv;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = k / nv;
+    const int v = (int)(k % nv);
+    const int j = (int)(c / nx), i = (int)(c % nx);
+    const int ip = (i == nx - 1) ? 0 : i + 1, jp = (j == ny - 1) ? 0 : j + 1;
+    const T ddx = dvd(fx[((size_t)j * nx + ip) * nv + v] - fx[k], dx);
+    const T ddy = dvd(fy[((size_t)jp * nx + i) * nv + v] - fy[k], dy);
+    out[k] = ddx + ddy;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) stress_faces_kernel(const T* tau, T* fx, T* fy,
+                                                                 int nx, int ny) {
+  const size_t n = (size_t)nx * ny;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(c / nx), i = (int)(c % nx);
+    const int im = (i == 0) ? nx - 1 : i - 1, jm = (j == 0) ? ny - 1 : j - 1;
+    const T* t0 = tau + c * 3;
+    const T* tw = tau + ((size_t)j * nx + im) * 3;
+    const T* ts = tau + ((size_t)jm * nx + i) * 3;
+    fx[c * 2 + 0] = T(0.5) * (tw[0] + t0[0]);
+    fx[c * 2 + 1] = T(0.5) * (tw[1] + t0[1]);
+    fy[c * 2 + 0] = T(0.5) * (ts[1] + t0[1]);
+    fy[c * 2 + 1] = T(0.5) * (ts[2] + t0[2]);
+  }
+}
+
+// Per-block float64 partial sums of each component (up to 8), fixed order.
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) box_sums_kernel(const T* u, size_t n, int nv,
+                                                             double* partial) {
+  __shared__ double s[8][kAuxBlock];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    for (int v = 0; v < nv; ++v) acc[v] += (double)u[c * nv + v];
+  for (int v = 0; v < 8; ++v) s[v][threadIdx.x] = acc[v];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int v = 0; v < 8; ++v) s[v][threadIdx.x] += s[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int v = 0; v < nv; ++v) partial[(size_t)blockIdx.x * 8 + v] = s[v][0];
+}
+
+__global__ void box_sums_finish_kernel(const double* partial, int nblocks, int nv, double* out) {
+  if (threadIdx.x != 0) return;
+  for (int v = 0; v < nv; ++v) {
+    double a = 0.0;
+    for (int b = 0; b < nblocks; ++b) a += partial[(size_t)b * 8 + v];
+    out[v] = a;
+  }
+}
+
+}  // namespace jx

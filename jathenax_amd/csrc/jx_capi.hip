@@ -1,0 +1,773 @@
+// C-ABI entry points: context, step and evolve orchestration, standalone operators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/jathenax_amd.h"
+#include "jx_aux.hpp"
+#include "jx_stage.hpp"
+
+using jx::Ctrl;
+using jx::StageArgs;
+
+// Stage launchers, one per (precision, reconstruction) translation unit.
+#define JX_DECL(T, NAME)                                                                    \
+  hipError_t NAME(int rs, int nv, bool visc, const jx::StageArgs<T>& a, dim3 grid, int bx, \
+                  hipStream_t st);
+JX_DECL(double, jx_stage_f64_dc)
+JX_DECL(double, jx_stage_f64_plm)
+JX_DECL(double, jx_stage_f64_ppm4)
+JX_DECL(double, jx_stage_f64_ppmx)
+JX_DECL(double, jx_stage_f64_wenoz)
+JX_DECL(float, jx_stage_f32_dc)
+JX_DECL(float, jx_stage_f32_plm)
+JX_DECL(float, jx_stage_f32_ppm4)
+JX_DECL(float, jx_stage_f32_ppmx)
+JX_DECL(float, jx_stage_f32_wenoz)
+#undef JX_DECL
+
+struct jx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Ctrl* ctrl = nullptr;       // device control block
+  Ctrl* host_ctrl = nullptr;  // pinned mirror for readback
+  void* buf[3] = {nullptr, nullptr, nullptr};   // ping-pong registers B, C and saved S
+  size_t buf_bytes = 0;
+  void* raw = nullptr;        // rendered raw force (evolve with forcing)
+  size_t raw_bytes = 0;
+  double* partial = nullptr;  // per-block partial sums
+  size_t partial_count = 0;
+  double* sums = nullptr;     // small device result vector
+  double* host_sums = nullptr;
+  int block_x = 0, rows = 0;
+  std::string err;
+  // Optional HIP-event timing of the fused stage launches.
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t events_used = 0;
+};
+
+namespace {
+
+int fail(jx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(jx_ctx* c, hipError_t e, const char* where) {
+  return fail(c, JX_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define JX_HIP(c, call)                                  \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail(c, e_, #call); \
+  } while (0)
+
+int ensure(jx_ctx* c, void** p, size_t* have, size_t need) {
+  if (*have >= need) return JX_OK;
+  if (*p) JX_HIP(c, hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  JX_HIP(c, hipMalloc(p, need));
+  *have = need;
+  return JX_OK;
+}
+
+int ensure_partial(jx_ctx* c, size_t count) {
+  size_t have = c->partial_count * sizeof(double);
+  void* p = c->partial;
+  int rc = ensure(c, &p, &have, count * sizeof(double));
+  c->partial = static_cast<double*>(p);
+  c->partial_count = have / sizeof(double);
+  return rc;
+}
+
+int ensure_buffers(jx_ctx* c, size_t bytes) {
+  if (c->buf_bytes >= bytes) return JX_OK;
+  for (auto& b : c->buf) {
+    if (b) JX_HIP(c, hipFree(b));
+    b = nullptr;
+  }
+  c->buf_bytes = 0;
+  for (auto& b : c->buf) JX_HIP(c, hipMalloc(&b, bytes));
+  c->buf_bytes = bytes;
+  return JX_OK;
+}
+
+dim3 aux_grid(size_t n) {
+  size_t b = (n + jx::kAuxBlock - 1) / jx::kAuxBlock;
+  b = std::max<size_t>(1, std::min<size_t>(b, 4096));
+  return dim3((unsigned)b);
+}
+
+// RK rows (jathenax/hydro.py:20-36).
+struct RkRow { double g0, g1, beta, delta; int has_delta; };
+const RkRow kRk2[2] = {{0.0, 1.0, 1.0, 0.0, 0}, {0.5, 0.5, 0.5, 0.0, 0}};
+const RkRow kRk3[3] = {{0.0, 1.0, 1.0, 0.0, 0},
+                       {0.25, 0.75, 0.25, 0.0, 0},
+                       {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0.0, 0}};
+const RkRow kRk4[4] = {{0.0, 1.0, 1.193743905974738, 0.0, 0},
+                       {0.121098479554482, 0.721781678111411, 0.099279895495783,
+                        0.217683334308543, 1},
+                       {-3.843833699660025, 2.121209265338722, 1.131678018054042,
+                        1.065841341361089, 1},
+                       {0.546370891121863, 0.198653035682705, 0.310665766509336, 0.0, 1}};
+
+const RkRow* rk_rows(int integrator) {
+  return integrator == 2 ? kRk2 : integrator == 3 ? kRk3 : integrator == 4 ? kRk4 : nullptr;
+}
+
+int check_params(jx_ctx* c, const jx_run_params* p) {
+  if (!p) return fail(c, JX_EINVAL, "params must not be NULL");
+  if (p->dtype != 0 && p->dtype != 1) return fail(c, JX_EINVAL, "dtype must be 0 (f64) or 1 (f32)");
+  if (p->nx < 1 || p->ny < 1) return fail(c, JX_EINVAL, "grid must be non-empty");
+  if (p->nvar < 3 || p->nvar > 5)
+    return fail(c, JX_EINVAL, "nvar must be 3 (hydro) plus at most two passive scalars");
+  if (p->reconstruct < 0 || p->reconstruct > 4)
+    return fail(c, JX_EKEY, "unknown reconstruction id");
+  if (p->rsolver < 0 || p->rsolver > 2) return fail(c, JX_EKEY, "unknown Riemann solver id");
+  if (!rk_rows(p->integrator)) return fail(c, JX_EKEY, "unknown integrator");
+  const double nonneg[3] = {p->viscosity, p->smagorinsky_cs, p->drag_rate};
+  const char* names[3] = {"viscosity", "smagorinsky_cs", "drag_rate"};
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(nonneg[k]) || nonneg[k] < 0.0)
+      return fail(c, JX_EINVAL, std::string(names[k]) + " must be finite and nonnegative");
+  return JX_OK;
+}
+
+template <typename T> struct StageLaunch;
+template <> struct StageLaunch<double> {
+  static hipError_t go(int recon, int rs, int nv, bool visc, const StageArgs<double>& a,
+                       dim3 g, int bx, hipStream_t st) {
+    switch (recon) {
+      case jx::RECON_DC: return jx_stage_f64_dc(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_PLM: return jx_stage_f64_plm(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_PPM4: return jx_stage_f64_ppm4(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_PPMX: return jx_stage_f64_ppmx(rs, nv, visc, a, g, bx, st);
+      default: return jx_stage_f64_wenoz(rs, nv, visc, a, g, bx, st);
+    }
+  }
+};
+template <> struct StageLaunch<float> {
+  static hipError_t go(int recon, int rs, int nv, bool visc, const StageArgs<float>& a,
+                       dim3 g, int bx, hipStream_t st) {
+    switch (recon) {
+      case jx::RECON_DC: return jx_stage_f32_dc(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_PLM: return jx_stage_f32_plm(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_PPM4: return jx_stage_f32_ppm4(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_PPMX: return jx_stage_f32_ppmx(rs, nv, visc, a, g, bx, st);
+      default: return jx_stage_f32_wenoz(rs, nv, visc, a, g, bx, st);
+    }
+  }
+};
+
+void pick_tiles(const jx_ctx* c, int nx, int ny, int* bx, int* rows) {
+  int b = c->block_x;
+  if (b <= 0) b = std::min(jx::kMaxBlock, 64 * ((nx + 6 + 63) / 64));
+  int r = c->rows;
+  if (r <= 0) r = 64;
+  *bx = b;
+  *rows = std::max(1, std::min(r, ny));
+}
+
+// Everything the stage launches of one step need.
+template <typename T> struct StepPlan {
+  const jx_run_params* p;
+  bool visc;
+  dim3 grid;
+  int bx, rows;
+  StageArgs<T> base;
+};
+
+template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p) {
+  StepPlan<T> s;
+  s.p = p;
+  s.visc = p->viscosity > 0.0 || p->smagorinsky_cs > 0.0;
+  pick_tiles(c, p->nx, p->ny, &s.bx, &s.rows);
+  s.grid = dim3((unsigned)((p->nx + (s.bx - 6) - 1) / (s.bx - 6)),
+                (unsigned)((p->ny + s.rows - 1) / s.rows));
+  StageArgs<T> a{};
+  a.ctrl = c->ctrl;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.rows = s.rows;
+  a.dx = T(p->dx);
+  a.dy = T(p->dy);
+  a.cs = T(p->cs);
+  a.cs2 = T(p->cs * p->cs);
+  a.inv_cs = T(1.0 / p->cs);
+  a.dfloor = T(p->dfloor);
+  a.eps = T(std::max(1e-42, (double)std::numeric_limits<T>::min()));
+  a.four_dx = T(4.0 * p->dx);
+  a.four_dy = T(4.0 * p->dy);
+  a.visc = T(p->viscosity);
+  a.smag_on = p->smagorinsky_cs > 0.0;
+  a.smag_c = T(p->smagorinsky_cs * p->smagorinsky_cs * p->dx * p->dy);
+  a.drag_on = p->drag_rate > 0.0;
+  a.drag = T(p->drag_rate);
+  s.base = a;
+  return s;
+}
+
+// Launch the RK stages of one step.  `first`: the saved register is the caller's input
+// (floored on read).  The result lands in `A`.  `raw`: kick the final stage.
+template <typename T>
+int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
+                const T* raw, bool want_dtkey) {
+  const jx_run_params* p = s.p;
+  const RkRow* rows = rk_rows(p->integrator);
+  const int ns = p->integrator;
+  T* B = static_cast<T*>(c->buf[0]);
+  T* C = static_cast<T*>(c->buf[1]);
+  T* S = static_cast<T*>(c->buf[2]);
+  const T* src = first ? in : A;
+  const size_t nblocks = (size_t)s.grid.x * s.grid.y;
+  if (raw) {
+    int rc = ensure_partial(c, std::max<size_t>(nblocks, 4096) * 3);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < ns; ++k) {
+    StageArgs<T> a = s.base;
+    const RkRow& r = rows[k];
+    a.gam0 = r.g0;
+    a.gam1 = r.g1;
+    a.beta = r.beta;
+    a.delta = r.delta;
+    a.has_delta = r.has_delta;
+    a.blended = (r.g0 != 0.0 && r.g0 + r.g1 == 1.0) ? 1 : 0;
+    a.stage1 = (k == 0);
+    a.floor_saved = 0;
+    a.u_sv = nullptr;
+    a.s_out = nullptr;
+    // Register plan (every read of u_sv / write of s_out is pointwise):
+    //   rk2: src->B, B|src->A          rk3: src->B, B|src->C, C|src->A
+    //   rk4: src->B, B|src->C (S=src+d2 B), C|S->B (S+=d3 C), B|S->A (S+=0 B)
+    if (k == 0) {
+      a.u_in = src;
+      a.u_out = B;
+    } else if (ns == 2) {
+      a.u_in = B;
+      a.u_out = A;
+    } else if (ns == 3) {
+      a.u_in = (k == 1) ? B : C;
+      a.u_out = (k == 1) ? C : A;
+    } else {
+      a.u_in = (k == 1) ? B : (k == 2) ? C : B;
+      a.u_out = (k == 1) ? C : (k == 2) ? B : A;
+      a.s_out = S;
+    }
+    if (k > 0) {
+      const bool from_src = (ns != 4) || (k == 1);
+      a.u_sv = from_src ? src : S;
+      a.floor_saved = (from_src && first) ? 1 : 0;
+    }
+    const bool last = (k == ns - 1);
+    a.raw = nullptr;
+    a.partial = nullptr;
+    a.dtkey = nullptr;
+    if (last && raw) {
+      a.raw = raw;
+      a.partial = c->partial;
+    } else if (last && want_dtkey) {
+      a.dtkey = &c->ctrl->dtkey;
+    }
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    if (c->timing) {
+      if (c->events_used == c->events.size()) {
+        std::pair<hipEvent_t, hipEvent_t> pr{};
+        JX_HIP(c, hipEventCreate(&pr.first));
+        JX_HIP(c, hipEventCreate(&pr.second));
+        c->events.push_back(pr);
+      }
+      ev = &c->events[c->events_used++];
+      JX_HIP(c, hipEventRecord(ev->first, c->stream));
+    }
+    hipError_t e = StageLaunch<T>::go(p->reconstruct, p->rsolver, p->nvar, s.visc, a, s.grid,
+                                      s.bx, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "stage kernel launch");
+    if (ev) JX_HIP(c, hipEventRecord(ev->second, c->stream));
+  }
+  if (raw) {
+    // Kick: forcing.py:137-149 then hydro.py:132, on the post-drag state in A.
+    const size_t n = (size_t)p->nx * p->ny;
+    hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+                       c->partial, (int)nblocks, 0, (double)n);
+    jx::KickArgs<T> ka{};
+    ka.u = A;
+    ka.raw = raw;
+    ka.nx = p->nx;
+    ka.ny = p->ny;
+    ka.nv = p->nvar;
+    ka.dfloor = T(p->dfloor);
+    ka.dx = T(p->dx);
+    ka.dy = T(p->dy);
+    ka.cs = T(p->cs);
+    ka.ctrl = c->ctrl;
+    ka.partial = c->partial;
+    const dim3 g = aux_grid(n);
+    hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       ka);
+    hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+                       c->partial, (int)g.x, 1, (double)n);
+    ka.dtkey = want_dtkey ? &c->ctrl->dtkey : nullptr;
+    hipLaunchKernelGGL(jx::kick_apply_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
+  }
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+template <typename T>
+jx::TimestepArgs<T> timestep_args(jx_ctx* c, const jx_run_params* p, const T* q, int primitive,
+                                  const Ctrl* ctrl) {
+  jx::TimestepArgs<T> t{};
+  t.q = q;
+  t.nx = p->nx;
+  t.ny = p->ny;
+  t.nv = p->nvar;
+  t.primitive = primitive;
+  t.smag_on = p->smagorinsky_cs > 0.0;
+  t.dx = T(p->dx);
+  t.dy = T(p->dy);
+  t.cs = T(p->cs);
+  t.dfloor = T(p->dfloor);
+  t.four_dx = T(4.0 * p->dx);
+  t.four_dy = T(4.0 * p->dy);
+  t.smag_c = T(p->smagorinsky_cs * p->smagorinsky_cs * p->dx * p->dy);
+  t.ctrl = ctrl;
+  t.dtkey = &c->ctrl->dtkey;
+  t.nukey = &c->ctrl->nukey;
+  return t;
+}
+
+template <typename T>
+int step_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, double dt, const T* raw,
+              double dedt) {
+  const size_t bytes = (size_t)p->nx * p->ny * p->nvar * sizeof(T);
+  int rc = ensure_buffers(c, bytes);
+  if (rc) return rc;
+  StepPlan<T> s = make_plan<T>(c, p);
+  hipLaunchKernelGGL(jx::set_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, dt, dedt);
+  return launch_step<T>(c, s, in, out, true, raw, false);
+}
+
+template <typename T>
+int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx_forcing* f,
+                double* t_out, double* dt_out, int* cyc_out) {
+  const size_t n = (size_t)p->nx * p->ny;
+  const size_t bytes = n * p->nvar * sizeof(T);
+  int rc = ensure_buffers(c, bytes);
+  if (rc) return rc;
+  const bool forced = f != nullptr;
+  if (forced) {
+    rc = ensure(c, &c->raw, &c->raw_bytes, n * 2 * sizeof(T));
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(jx::init_ctrl_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl);
+  if (!(0.0 < p->tlim && 0 < p->nlim)) {
+    JX_HIP(c, hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+    JX_HIP(c, hipStreamSynchronize(c->stream));
+    *t_out = 0.0;
+    *dt_out = INFINITY;
+    *cyc_out = 0;
+    return JX_OK;
+  }
+  StepPlan<T> s = make_plan<T>(c, p);
+  const bool smag = p->smagorinsky_cs > 0.0;
+  jx::LoopArgs la{};
+  la.tlim = p->tlim;
+  la.cfl = p->cfl;
+  la.viscosity = p->viscosity;
+  la.drag_rate = p->drag_rate;
+  la.h = std::min(p->dx, p->dy);
+  la.dt_update = p->dt_update;
+  la.tdriv_start = p->tdriv_start;
+  la.dedt = forced ? p->dedt : 0.0;
+  la.nlim = p->nlim;
+  la.smag_on = smag;
+  la.has_forcing = forced;
+  la.n_updates = forced ? f->n_updates : 0;
+  jx::RenderArgs<T> ra{};
+  if (forced) {
+    ra.raw = static_cast<T*>(c->raw);
+    ra.table = static_cast<const T*>(f->coefficients);
+    ra.cx = static_cast<const T*>(f->cos_x);
+    ra.sx = static_cast<const T*>(f->sin_x);
+    ra.cy = static_cast<const T*>(f->cos_y);
+    ra.sy = static_cast<const T*>(f->sin_y);
+    ra.nx = p->nx;
+    ra.ny = p->ny;
+    ra.n_modes = f->n_modes;
+    ra.n_updates = f->n_updates;
+    ra.ctrl = c->ctrl;
+    JX_HIP(c, hipMemsetAsync(c->raw, 0, n * 2 * sizeof(T), c->stream));
+  }
+  const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
+  // Initial timestep from the caller's state (run.py:42-46).
+  hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0, c->stream,
+                     timestep_args<T>(c, p, in, 0, c->ctrl));
+  auto one_step = [&](bool first) -> int {
+    hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
+    if (forced)
+      hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                         c->stream, ra);
+    int r = launch_step<T>(c, s, in, out, first, raw, !smag);
+    if (r) return r;
+    if (smag)
+      hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                         c->stream, timestep_args<T>(c, p, out, 0, c->ctrl));
+    return JX_OK;
+  };
+  rc = one_step(true);
+  if (rc) return rc;
+  long long chunk = 1;
+  for (;;) {
+    hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
+    JX_HIP(c, hipMemcpyAsync(c->host_ctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                             c->stream));
+    JX_HIP(c, hipStreamSynchronize(c->stream));
+    const Ctrl h = *c->host_ctrl;
+    if (h.done) {
+      *t_out = h.time;
+      *dt_out = h.prev_dt;
+      *cyc_out = h.cycle;
+      return JX_OK;
+    }
+    // The begin_step above already opened the next step: launch it, then a chunk more.
+    // Remaining steps are estimated from the time left; dt can shrink, so loop.
+    double est = (p->tlim - h.time) / h.dt;
+    long long want = std::isfinite(est) ? (long long)std::ceil(est) : 1;
+    want = std::min<long long>(want, (long long)p->nlim - h.cycle);
+    chunk = std::max<long long>(1, std::min<long long>(want, 512));
+    {
+      if (forced)
+        hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                           c->stream, ra);
+      int r = launch_step<T>(c, s, in, out, false, raw, !smag);
+      if (r) return r;
+      if (smag)
+        hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                           c->stream, timestep_args<T>(c, p, out, 0, c->ctrl));
+    }
+    for (long long k = 1; k < chunk; ++k) {
+      rc = one_step(false);
+      if (rc) return rc;
+    }
+  }
+}
+
+template <typename T>
+int normalize_impl(jx_ctx* c, int ny, int nx, int nvar, const T* U, const T* raw, double dt,
+                          double dedt, T* acc) {
+  const size_t n = (size_t)nx * ny;
+  const dim3 g = aux_grid(n);
+  int rc = ensure_partial(c, (size_t)g.x * 3);
+  if (rc) return rc;
+  hipLaunchKernelGGL(jx::set_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, dt, dedt);
+  jx::KickArgs<T> ka{};
+  ka.u = const_cast<T*>(U);
+  ka.raw = raw;
+  ka.nx = nx;
+  ka.ny = ny;
+  ka.nv = nvar;
+  ka.ctrl = c->ctrl;
+  ka.partial = c->partial;
+  hipLaunchKernelGGL(jx::kick_first_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
+  hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+                     c->partial, (int)g.x, 0, (double)n);
+  hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
+  hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+                     c->partial, (int)g.x, 1, (double)n);
+  hipLaunchKernelGGL(jx::kick_acceleration_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                     (const Ctrl*)c->ctrl, raw, acc, n);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jx_ctx_create(jx_ctx** out, int device, void* stream) {
+  if (!out) return JX_EINVAL;
+  *out = nullptr;
+  jx_ctx* c = new jx_ctx();
+  c->device = device;
+  c->stream = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMalloc(&c->ctrl, sizeof(Ctrl));
+  if (e == hipSuccess) e = hipHostMalloc(&c->host_ctrl, sizeof(Ctrl), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&c->sums, 16 * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&c->host_sums, 16 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "jx_ctx_create: %s\n", hipGetErrorString(e));
+    jx_ctx_destroy(c);
+    return JX_EHIP;
+  }
+  *out = c;
+  return JX_OK;
+}
+
+void jx_ctx_destroy(jx_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  else (void)hipDeviceSynchronize();
+  for (auto& b : c->buf)
+    if (b) (void)hipFree(b);
+  if (c->raw) (void)hipFree(c->raw);
+  if (c->partial) (void)hipFree(c->partial);
+  if (c->ctrl) (void)hipFree(c->ctrl);
+  if (c->sums) (void)hipFree(c->sums);
+  if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
+  if (c->host_sums) (void)hipHostFree(c->host_sums);
+  for (auto& pr : c->events) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  delete c;
+}
+
+const char* jx_last_error(const jx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int jx_set_tiles(jx_ctx* c, int block_x, int rows) {
+  if (!c) return JX_EINVAL;
+  if (block_x != 0 && (block_x < 64 || block_x > jx::kMaxBlock || block_x % 64 != 0))
+    return fail(c, JX_EINVAL, "block_x must be 0 or a multiple of 64 in [64, 256]");
+  if (rows < 0) return fail(c, JX_EINVAL, "rows must be >= 0");
+  c->block_x = block_x;
+  c->rows = rows;
+  return JX_OK;
+}
+
+int jx_set_kernel_timing(jx_ctx* c, int on) {
+  if (!c) return JX_EINVAL;
+  c->timing = on != 0;
+  c->events_used = 0;
+  return JX_OK;
+}
+
+int jx_kernel_timing(jx_ctx* c, int* launches, double* total_ms) {
+  if (!c || !launches || !total_ms) return JX_EINVAL;
+  JX_HIP(c, hipStreamSynchronize(c->stream));
+  double total = 0.0;
+  int n = 0;
+  for (size_t k = 0; k < c->events_used; ++k) {
+    float ms = 0.0f;
+    hipError_t e = hipEventElapsedTime(&ms, c->events[k].first, c->events[k].second);
+    if (e != hipSuccess) return hip_fail(c, e, "hipEventElapsedTime");
+    total += ms;
+    ++n;
+  }
+  c->events_used = 0;
+  *launches = n;
+  *total_ms = total;
+  return JX_OK;
+}
+
+int jx_isothermal_step(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
+                       double dt, const void* raw_force, double dedt) {
+  if (!c) return JX_EINVAL;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!U_in || !U_out || U_in == U_out)
+    return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
+  if (p->dtype == 0)
+    return step_impl<double>(c, p, static_cast<const double*>(U_in), static_cast<double*>(U_out),
+                             dt, static_cast<const double*>(raw_force), dedt);
+  return step_impl<float>(c, p, static_cast<const float*>(U_in), static_cast<float*>(U_out), dt,
+                          static_cast<const float*>(raw_force), dedt);
+}
+
+int jx_isothermal_timestep(jx_ctx* c, const jx_run_params* p, const void* W, double previous_dt,
+                           double time, double tlim, double* dt_out) {
+  if (!c || !dt_out) return JX_EINVAL;
+  if (!p || !W) return fail(c, JX_EINVAL, "params and W must not be NULL");
+  if (p->nx < 1 || p->ny < 1 || p->nvar < 3) return fail(c, JX_EINVAL, "bad shape");
+  const size_t n = (size_t)p->nx * p->ny;
+  hipLaunchKernelGGL(jx::init_ctrl_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl);
+  if (p->dtype == 0)
+    hipLaunchKernelGGL(jx::timestep_kernel<double>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                       c->stream,
+                       timestep_args<double>(c, p, static_cast<const double*>(W), 1, nullptr));
+  else
+    hipLaunchKernelGGL(jx::timestep_kernel<float>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                       c->stream,
+                       timestep_args<float>(c, p, static_cast<const float*>(W), 1, nullptr));
+  JX_HIP(c, hipGetLastError());
+  JX_HIP(c, hipMemcpyAsync(c->host_ctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                           c->stream));
+  JX_HIP(c, hipStreamSynchronize(c->stream));
+  const Ctrl h = *c->host_ctrl;
+  *dt_out = jx::finish_timestep(jx::value_of_key(h.dtkey), jx::value_of_max_key(h.nukey),
+                                p->smagorinsky_cs > 0.0, p->viscosity, p->drag_rate,
+                                std::min(p->dx, p->dy), p->cfl, previous_dt, time, tlim);
+  return JX_OK;
+}
+
+int jx_evolve_isothermal(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
+                         const jx_forcing* f, double* time_out, double* dt_out, int* cycle_out) {
+  if (!c || !time_out || !dt_out || !cycle_out) return JX_EINVAL;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!std::isfinite(p->dedt) || p->dedt < 0.0)
+    return fail(c, JX_EINVAL, "dedt must be finite and nonnegative");
+  if (f && (!(p->dt_update > 0.0) || !std::isfinite(p->dt_update) ||
+            !std::isfinite(p->tdriv_start)))
+    return fail(c, JX_EINVAL, "dt_update must be finite and positive and tdriv_start finite");
+  if (f && (!f->coefficients || !f->cos_x || !f->sin_x || !f->cos_y || !f->sin_y ||
+            f->n_modes < 1 || f->n_updates < 1))
+    return fail(c, JX_EINVAL, "incomplete forcing tables");
+  if (!U_in || !U_out || U_in == U_out)
+    return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
+  if (p->dtype == 0)
+    return evolve_impl<double>(c, p, static_cast<const double*>(U_in),
+                               static_cast<double*>(U_out), f, time_out, dt_out, cycle_out);
+  return evolve_impl<float>(c, p, static_cast<const float*>(U_in), static_cast<float*>(U_out), f,
+                            time_out, dt_out, cycle_out);
+}
+
+int jx_recover_isothermal(jx_ctx* c, int dtype, long long ncells, int nvar, const void* U,
+                          double dfloor, void* U_out, void* W_out) {
+  if (!c) return JX_EINVAL;
+  if (ncells < 0 || nvar < 1 || !U) return fail(c, JX_EINVAL, "bad recover arguments");
+  const dim3 g = aux_grid((size_t)ncells);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::recover_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const double*>(U), static_cast<double*>(U_out),
+                       static_cast<double*>(W_out), (size_t)ncells, nvar, dfloor);
+  else
+    hipLaunchKernelGGL(jx::recover_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const float*>(U), static_cast<float*>(U_out),
+                       static_cast<float*>(W_out), (size_t)ncells, nvar, (float)dfloor);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_rk_stage(jx_ctx* c, int dtype, long long n, const void* U, const void* U_saved,
+                const void* div_flux, double dt, double gam0, double gam1, double beta,
+                double delta, int has_delta, void* U_next, void* U_saved_out) {
+  if (!c) return JX_EINVAL;
+  if (n < 0 || !U || !U_saved || !div_flux || !U_next)
+    return fail(c, JX_EINVAL, "bad rk_stage arguments");
+  jx::StageCoef k{gam0, gam1, beta, delta, has_delta, (gam0 != 0.0 && gam0 + gam1 == 1.0)};
+  const dim3 g = aux_grid((size_t)n);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::rk_stage_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const double*>(U), static_cast<const double*>(U_saved),
+                       static_cast<const double*>(div_flux), static_cast<double*>(U_next),
+                       static_cast<double*>(U_saved_out), (size_t)n, dt, k);
+  else
+    hipLaunchKernelGGL(jx::rk_stage_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const float*>(U), static_cast<const float*>(U_saved),
+                       static_cast<const float*>(div_flux), static_cast<float*>(U_next),
+                       static_cast<float*>(U_saved_out), (size_t)n, dt, k);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_flux_divergence(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* fx,
+                       const void* fy, double dx, double dy, void* out) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 1 || !fx || !fy || !out)
+    return fail(c, JX_EINVAL, "bad flux_divergence arguments");
+  const dim3 g = aux_grid((size_t)nx * ny * nvar);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::flux_divergence_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const double*>(fx), static_cast<const double*>(fy),
+                       static_cast<double*>(out), nx, ny, nvar, dx, dy);
+  else
+    hipLaunchKernelGGL(jx::flux_divergence_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const float*>(fx), static_cast<const float*>(fy),
+                       static_cast<float*>(out), nx, ny, nvar, (float)dx, (float)dy);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_stress_face_fluxes(jx_ctx* c, int dtype, int ny, int nx, const void* tau, void* fx,
+                          void* fy) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || !tau || !fx || !fy)
+    return fail(c, JX_EINVAL, "Expected stress with shape (ny, nx, 3)");
+  const dim3 g = aux_grid((size_t)nx * ny);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::stress_faces_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const double*>(tau), static_cast<double*>(fx),
+                       static_cast<double*>(fy), nx, ny);
+  else
+    hipLaunchKernelGGL(jx::stress_faces_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const float*>(tau), static_cast<float*>(fx),
+                       static_cast<float*>(fy), nx, ny);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_render_force(jx_ctx* c, int dtype, int ny, int nx, int n_modes, const void* amplitudes,
+                    const void* cos_x, const void* sin_x, const void* cos_y, const void* sin_y,
+                    void* raw_out) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || n_modes < 0 || !raw_out || (n_modes > 0 && !amplitudes))
+    return fail(c, JX_EINVAL, "bad render_force arguments");
+  const dim3 g = aux_grid((size_t)nx * ny);
+  if (dtype == 0) {
+    jx::RenderArgs<double> a{static_cast<double*>(raw_out), static_cast<const double*>(amplitudes),
+                             static_cast<const double*>(cos_x), static_cast<const double*>(sin_x),
+                             static_cast<const double*>(cos_y), static_cast<const double*>(sin_y),
+                             nx, ny, n_modes, 1, nullptr, 0};
+    hipLaunchKernelGGL(jx::render_force_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream, a);
+  } else {
+    jx::RenderArgs<float> a{static_cast<float*>(raw_out), static_cast<const float*>(amplitudes),
+                            static_cast<const float*>(cos_x), static_cast<const float*>(sin_x),
+                            static_cast<const float*>(cos_y), static_cast<const float*>(sin_y),
+                            nx, ny, n_modes, 1, nullptr, 0};
+    hipLaunchKernelGGL(jx::render_force_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream, a);
+  }
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_normalize_force(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* U,
+                       const void* raw_force, double dt, double dedt, void* acc_out) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 3 || !U || !raw_force || !acc_out)
+    return fail(c, JX_EINVAL, "bad normalize_force arguments");
+  if (dtype == 0)
+    return normalize_impl<double>(c, ny, nx, nvar, static_cast<const double*>(U),
+                                  static_cast<const double*>(raw_force), dt, dedt,
+                                  static_cast<double*>(acc_out));
+  return normalize_impl<float>(c, ny, nx, nvar, static_cast<const float*>(U),
+                               static_cast<const float*>(raw_force), dt, dedt,
+                               static_cast<float*>(acc_out));
+}
+
+int jx_conserved_sums(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* U,
+                      double* sums_out) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 1 || nvar > 8 || !U || !sums_out)
+    return fail(c, JX_EINVAL, "bad conserved_sums arguments");
+  const size_t n = (size_t)nx * ny;
+  const dim3 g = aux_grid(n);
+  int rc = ensure_partial(c, (size_t)g.x * 8);
+  if (rc) return rc;
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::box_sums_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const double*>(U), n, nvar, c->partial);
+  else
+    hipLaunchKernelGGL(jx::box_sums_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const float*>(U), n, nvar, c->partial);
+  hipLaunchKernelGGL(jx::box_sums_finish_kernel, dim3(1), dim3(64), 0, c->stream, c->partial,
+                     (int)g.x, nvar, c->sums);
+  JX_HIP(c, hipGetLastError());
+  JX_HIP(c, hipMemcpyAsync(c->host_sums, c->sums, nvar * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  JX_HIP(c, hipStreamSynchronize(c->stream));
+  std::memcpy(sums_out, c->host_sums, nvar * sizeof(double));
+  return JX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+// Scalar face operators of the isothermal update, shared by every gfx950 kernel.
+//
+// Each function restates one reference operator (jathenax/fluxes.py) for one cell or
+// face, in the reference's floating-point expression order.  The float64 translation
+// units are built with -ffp-contract=off, so in FP64 these are the IEEE evaluations of
+// the reference expressions.  In FP32 (Fast<float>) divisions by run constants become
+// reciprocal multiplies and sqrt/rcp use the hardware approximations; FP32 results are
+// checked against the FP64 form by error norms, not bit patterns.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+#include "jx_types.hpp"
+
+namespace jx {
+
+template <typename T> struct Fast { static constexpr bool value = false; };
+template <> struct Fast<float> { static constexpr bool value = true; };
+
+// a / b: IEEE division in FP64, v_rcp_f32-based in FP32.
+template <typename T> __device__ __forceinline__ T dvd(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float dvd<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+// x / c for a compile-time constant c (the reference's "/ 12.0", "/ 6.0").
+template <typename T> __device__ __forceinline__ T divk(T x, double c) {
+  if constexpr (Fast<T>::value) return x * T(1.0 / c);
+  else return x / T(c);
+}
+template <typename T> __device__ __forceinline__ T sqr(T x) { return x * x; }
+template <typename T> __device__ __forceinline__ T root(T x) { return sqrt(x); }
+template <> __device__ __forceinline__ float root<float>(float x) { return __builtin_sqrtf(x); }
+template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return fmin(a, b); }
+template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return fmax(a, b); }
+
+// ---------------------------------------------------------------------------------------
+// Reconstruction: low and high edge values of one cell from its 5-point stencil.  The
+// reference's left state at face i is the high edge of cell i-1 and its right state the
+// low edge of cell i (fluxes.py:6-117).
+// ---------------------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T ppmx_limit_edge(T a, T b, T e, T ca, T cb) {
+  // fluxes.py:45-53
+  const T curv = T(3) * ((a + b) - T(2) * e);
+  const bool same = ((curv > T(0)) && (ca > T(0)) && (cb > T(0))) ||
+                    ((curv < T(0)) && (ca < T(0)) && (cb < T(0)));
+  const T mag = fmn(fabs(ca), fabs(cb));
+  const T lim = copysign(fmn(T(1.25) * mag, fabs(curv)), curv);
+  const T corr = T(0.5) * (a + b) - divk(same ? lim : T(0), 6.0);
+  return ((a - e) * (b - e) > T(0)) ? corr : e;
+}
+
+template <int RECON, typename T>
+__device__ __forceinline__ void cell_edges(T qm2, T qm1, T q, T qp1, T qp2, T eps, T& lo,
+                                           T& hi) {
+  if constexpr (RECON == RECON_DC) {
+    // fluxes.py:6-8
+    lo = q;
+    hi = q;
+  } else if constexpr (RECON == RECON_PLM) {
+    // fluxes.py:11-18
+    const T dl = q - qm1, dr = qp1 - q;
+    const T p = dl * dr;
+    const T s = (p > T(0)) ? dvd(p, dl + dr) : T(0);
+    hi = q + s;
+    lo = q - s;
+  } else if constexpr (RECON == RECON_PPM4) {
+    // fluxes.py:21-42
+    T l = divk(T(7) * (q + qm1) - (qm2 + qp1), 12.0);
+    T h = divk(T(7) * (q + qp1) - (qm1 + qp2), 12.0);
+    l = fmn(fmx(l, fmn(q, qm1)), fmx(q, qm1));
+    h = fmn(fmx(h, fmn(q, qp1)), fmx(q, qp1));
+    const T dh = h - q, dl = l - q;
+    const bool ext = dh * dl >= T(0);
+    const T l2 = (fabs(dl) >= T(2) * fabs(dh)) ? q - T(2) * dh : l;
+    const T h2 = (fabs(dh) >= T(2) * fabs(dl)) ? q - T(2) * dl : h;
+    lo = ext ? q : l2;
+    hi = ext ? q : h2;
+  } else if constexpr (RECON == RECON_PPMX) {
+    // fluxes.py:56-84
+    T l = divk(T(7) * (q + qm1) - (qm2 + qp1), 12.0);
+    T h = divk(T(7) * (q + qp1) - (qm1 + qp2), 12.0);
+    const T d2l = (qm2 + q) - T(2) * qm1;
+    const T d2c = (qm1 + qp1) - T(2) * q;
+    const T d2r = (q + qp2) - T(2) * qp1;
+    l = ppmx_limit_edge(qm1, q, l, d2l, d2c);
+    h = ppmx_limit_edge(q, qp1, h, d2c, d2r);
+    const T dh = h - q, dl = l - q;
+    const bool ext = (dh * (q - l) <= T(0)) || ((qm1 - q) * (q - qp1) <= T(0));
+    const T d2 = T(6) * ((l + h) - T(2) * q);
+    const bool same = ((d2 > T(0)) && (d2l > T(0)) && (d2c > T(0)) && (d2r > T(0))) ||
+                      ((d2 < T(0)) && (d2l < T(0)) && (d2c < T(0)) && (d2r < T(0)));
+    const T mag = fmn(fabs(d2c), fmn(fabs(d2l), fabs(d2r)));
+    const T lim = copysign(fmn(T(1.25) * mag, fabs(d2)), d2);
+    const T scale = fmx(fabs(qm1), fmx(fabs(q), fabs(qp1)));
+    const bool use = same && (fabs(d2) > T(1e-12) * scale);
+    const T ratio = use ? dvd(lim, d2) : T(0);
+    const T lm = (fabs(dl) >= T(2) * fabs(dh)) ? q - T(2) * dh : l;
+    const T hm = (fabs(dh) >= T(2) * fabs(dl)) ? q - T(2) * dl : h;
+    lo = ext ? q + dl * ratio : lm;
+    hi = ext ? q + dh * ratio : hm;
+  } else {
+    // WENO-Z, fluxes.py:87-117
+    const T c13 = T(13.0 / 12.0);
+    const T b0 = c13 * sqr((qm2 + q) - T(2) * qm1) + T(0.25) * sqr((qm2 + T(3) * q) - T(4) * qm1);
+    const T b1 = c13 * sqr((qm1 + qp1) - T(2) * q) + T(0.25) * sqr(qm1 - qp1);
+    const T b2 = c13 * sqr((qp2 + q) - T(2) * qp1) + T(0.25) * sqr((qp2 + T(3) * q) - T(4) * qp1);
+    const T tau = fabs(b0 - b2);
+    const T d0 = b0 + eps, d1 = b1 + eps, d2 = b2 + eps;
+    const T dmin = fmn(d0, fmn(d1, d2));
+    const T scale = fmx(tau, dmin);
+    const T base = sqr(dvd(dmin, scale));
+    const T contrast = dvd(tau, scale);
+    const T w0 = base + sqr(contrast * dvd(dmin, d0));
+    const T w1 = base + sqr(contrast * dvd(dmin, d1));
+    const T w2 = base + sqr(contrast * dvd(dmin, d2));
+    {
+      const T a0 = T(0.1) * w0, a1 = T(0.6) * w1, a2 = T(0.3) * w2;
+      const T f0 = (T(2) * qm2 - T(7) * qm1) + T(11) * q;
+      const T f1 = (-qm1 + T(5) * q) + T(2) * qp1;
+      const T f2 = (T(2) * q + T(5) * qp1) - qp2;
+      hi = dvd((f0 * a0 + f1 * a1) + f2 * a2, T(6) * ((a0 + a1) + a2));
+    }
+    {
+      const T a0 = T(0.1) * w2, a1 = T(0.6) * w1, a2 = T(0.3) * w0;
+      const T f0 = (T(2) * qp2 - T(7) * qp1) + T(11) * q;
+      const T f1 = (-qp1 + T(5) * q) + T(2) * qm1;
+      const T f2 = (T(2) * q + T(5) * qm1) - qm2;
+      lo = dvd((f0 * a0 + f1 * a1) + f2 * a2, T(6) * ((a0 + a1) + a2));
+    }
+  }
+}
+
+// PPMX and WENO-Z floor the face density (fluxes.py:128-130).
+template <int RECON> __host__ __device__ constexpr bool floors_faces() {
+  return RECON == RECON_PPMX || RECON == RECON_WENOZ;
+}
+
+// ---------------------------------------------------------------------------------------
+// Isothermal Riemann solvers in (normal, tangent) order: returns mass, normal momentum and
+// tangential momentum fluxes (fluxes.py:134-201).
+// ---------------------------------------------------------------------------------------
+
+template <typename T> struct Gas {
+  T cs, cs2, inv_cs;
+};
+
+template <int RS, typename T>
+__device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
+                                        T& f0, T& f1, T& f2) {
+  const T cs = g.cs, cs2 = g.cs2;
+  if constexpr (RS == RS_LLF) {
+    // fluxes.py:175-183
+    const T mnl = rl * ul, mnr = rr * ur;
+    const T mtl = rl * tl, mtr = rr * tr;
+    const T s0 = mnl + mnr;
+    const T s1 = (mnl * ul + mnr * ur) + cs2 * (rl + rr);
+    const T s2 = mnl * tl + mnr * tr;
+    const T sp = fmx(fabs(ul) + cs, fabs(ur) + cs);
+    f0 = T(0.5) * (s0 - sp * (rr - rl));
+    f1 = T(0.5) * (s1 - sp * (mnr - mnl));
+    f2 = T(0.5) * (s2 - sp * (mtr - mtl));
+  } else if constexpr (RS == RS_HLLE) {
+    // fluxes.py:186-201
+    const T sl = root(rl), sr = root(rr);
+    const T vn = (sl * ul + sr * ur) * dvd(T(1), sl + sr);
+    const T al = fmn(vn - cs, ul - cs);
+    const T ar = fmx(vn + cs, ur + cs);
+    const T bm = (al < T(0)) ? al : T(-1e-20);
+    const T bp = (ar > T(0)) ? ar : T(1e-20);
+    const T dl = ul - bm, dr = ur - bp;
+    const T fl0 = rl * dl, fl1 = (rl * ul) * dl + cs2 * rl, fl2 = (rl * tl) * dl;
+    const T fr0 = rr * dr, fr1 = (rr * ur) * dr + cs2 * rr, fr2 = (rr * tr) * dr;
+    const T wt = dvd(T(0.5) * (bp + bm), bp - bm);
+    f0 = T(0.5) * (fl0 + fr0) + wt * (fl0 - fr0);
+    f1 = T(0.5) * (fl1 + fr1) + wt * (fl1 - fr1);
+    f2 = T(0.5) * (fl2 + fr2) + wt * (fl2 - fr2);
+  } else {
+    // Roe with supersonic upwinding and the LLF fallback, fluxes.py:134-172
+    const T mnl = rl * ul, mnr = rr * ur;
+    const T mtl = rl * tl, mtr = rr * tr;
+    const T fl0 = mnl, fl1 = mnl * ul + cs2 * rl, fl2 = mnl * tl;
+    const T fr0 = mnr, fr1 = mnr * ur + cs2 * rr, fr2 = mnr * tr;
+    const T fa0 = T(0.5) * (fl0 + fr0), fa1 = T(0.5) * (fl1 + fr1), fa2 = T(0.5) * (fl2 + fr2);
+    const T sl = root(rl), sr = root(rr);
+    const T inv = dvd(T(1), sl + sr);
+    const T vn = (sl * ul + sr * ur) * inv;
+    const T vt = (sl * tl + sr * tr) * inv;
+    const T wm = vn - cs, wp = vn + cs;
+    const T drho = rr - rl, dmn = mnr - mnl, dmt = mtr - mtl;
+    T am, ap;
+    if constexpr (Fast<T>::value) {
+      am = drho * (T(0.5) + T(0.5) * vn * g.inv_cs) - dmn * T(0.5) * g.inv_cs;
+      ap = drho * (T(0.5) - T(0.5) * vn * g.inv_cs) + dmn * T(0.5) * g.inv_cs;
+    } else {
+      am = drho * (T(0.5) + (T(0.5) * vn) / cs) - (dmn * T(0.5)) / cs;
+      ap = drho * (T(0.5) - (T(0.5) * vn) / cs) + (dmn * T(0.5)) / cs;
+    }
+    const T ash = drho * (-vt) + dmt;
+    const T cm = (T(-0.5) * fabs(wm)) * am;
+    const T csh = (T(-0.5) * fabs(vn)) * ash;
+    const T cp = (T(-0.5) * fabs(wp)) * ap;
+    T m0 = (fa0 + cm) + cp;
+    T m1 = (fa1 + cm * wm) + cp * wp;
+    T m2 = ((fa2 + cm * vt) + csh) + cp * vt;
+    if (wm >= T(0)) { m0 = fl0; m1 = fl1; m2 = fl2; }
+    if (wp <= T(0)) { m0 = fr0; m1 = fr1; m2 = fr2; }
+    const T rstar = rl + am;
+    if ((rstar < T(0)) || (rstar + ap < T(0))) {
+      const T sp = T(0.5) * fmx(fabs(ul) + cs, fabs(ur) + cs);
+      m0 = fa0 - sp * drho;
+      m1 = fa1 - sp * dmn;
+      m2 = fa2 - sp * dmt;
+    }
+    f0 = m0;
+    f1 = m1;
+    f2 = m2;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Viscous momentum flux at one face (fluxes.py:227-266).  Inputs are the face velocity
+// gradients dvx/dx, dvy/dy, dvx/dy, dvy/dx and rho_i + rho_(i-1).  Returns the x and y
+// momentum flux increments -mu * stress.  AXIS follows the reference: 1 = x faces.
+// ---------------------------------------------------------------------------------------
+
+template <typename T> struct Visc {
+  T nu;       // uniform kinematic viscosity
+  T smag_c;   // Cs^2 dx dy (0: Smagorinsky off)
+  bool smag;
+};
+
+template <typename T>
+__device__ __forceinline__ T smag_nu(T ux, T vy, T shear, T c) {
+  // fluxes.py:241-245
+  const T s2 = T(2) * (ux * ux + vy * vy) + shear * shear;
+  return c * root(s2);
+}
+
+template <int AXIS, typename T>
+__device__ __forceinline__ void visc_face(T ux, T vy, T uy, T vx, T rho_sum, const Visc<T>& p,
+                                          T& fmx_out, T& fmy_out) {
+  const T shear = vx + uy;    // dv_dx[..., 1] + dv_dy[..., 0]
+  T sx, sy;
+  if constexpr (AXIS == 1) {
+    sx = T(4.0 / 3.0) * ux - T(2.0 / 3.0) * vy;
+    sy = shear;
+  } else {
+    sx = shear;
+    sy = T(4.0 / 3.0) * vy - T(2.0 / 3.0) * ux;
+  }
+  T nu = p.nu;
+  if (p.smag) nu = p.nu + smag_nu(ux, vy, shear, p.smag_c);
+  const T mu = (T(0.5) * nu) * rho_sum;
+  fmx_out = (-mu) * sx;
+  fmy_out = (-mu) * sy;
+}
+
+// ---------------------------------------------------------------------------------------
+// Order-preserving keys for the grid-wide dt min / nu max (NaN maps to 0 so that it wins
+// a min, mirroring jnp.min's NaN propagation).
+// ---------------------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned long long key_of(double x) {
+  if (x != x) return 0ull;
+  return static_cast<unsigned long long>(__double_as_longlong(x)) + 1ull;
+}
+__host__ __device__ inline double value_of_key(unsigned long long k) {
+  if (k == 0ull) return __builtin_nan("");
+  union { unsigned long long u; double d; } c;
+  c.u = k - 1ull;
+  return c.d;
+}
+
+}  // namespace jx

@@ -1,0 +1,366 @@
+// The fused RK-stage kernel: one launch per stage of jathenax/hydro.py:101-143.
+//
+// Decomposition.  A block of BX threads owns a strip of BX-6 output columns and `rows`
+// output rows.  Thread t follows column x0+t-3 (periodic) and marches up the strip one
+// row per iteration, so every input row is read from HBM once per strip:
+//   * y direction: the thread keeps its column's last five primitive rows in registers
+//     and builds the y-face states, the y Riemann flux and (with viscosity) the y-face
+//     viscous flux without touching LDS for the stencil;
+//   * x direction: the row being updated is staged in LDS; the thread reconstructs its
+//     own cell, exchanges high edges and face fluxes through LDS (three barriers per row)
+//     and forms the x flux difference.  The 3+3 halo columns are computed, not stored.
+// Face states and fluxes never leave the CU.  The update applies the RK row
+// (hydro.py:39-51), drag with the incoming primitives (hydro.py:122-125), the density
+// floor (hydro.py:54-57) and, in the epilogue, either the CFL min for the next timestep
+// (hydro.py:66-68) or the first moments of the forcing normalisation (forcing.py:141).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jx_ops.hpp"
+
+namespace jx {
+
+constexpr int kMaxBlock = 256;   // threads per stage block (launch bound)
+
+template <typename T> struct StageArgs {
+  const T* __restrict__ u_in;   // stage input, read with halos (never aliases u_out)
+  const T* u_sv;                // saved register, read pointwise (may alias u_out/s_out)
+  T* u_out;                     // stage output
+  T* s_out;                     // RK4 saved register output (nullptr when no delta)
+  const T* __restrict__ raw;    // raw force (ny, nx, 2) when this stage feeds the kick
+  const Ctrl* ctrl;             // dt of the step and the done flag
+  unsigned long long* dtkey;    // CFL-limit min slot (nullptr: no epilogue)
+  double* partial;              // kick moments per block [3] (nullptr: no kick)
+  int nx, ny, rows;
+  int stage1;                   // saved register is the (floored) stage input itself
+  int floor_saved;              // apply the density floor when reading u_sv
+  int has_delta, blended, drag_on, smag_on;
+  T dx, dy, cs, cs2, inv_cs, dfloor, eps, four_dx, four_dy, visc, smag_c, drag;
+  double gam0, gam1, beta, delta;
+};
+
+template <typename T>
+__device__ __forceinline__ void block_reduce_store(double v0, double v1, double v2,
+                                                   double* sred, double* out) {
+  // Fixed butterfly inside each wave, then waves in index order: deterministic.
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off);
+    v1 += __shfl_xor(v1, off);
+    v2 += __shfl_xor(v2, off);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sred[3 * wave + 0] = v0;
+    sred[3 * wave + 1] = v1;
+    sred[3 * wave + 2] = v2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) {
+      a += sred[3 * w + 0];
+      b += sred[3 * w + 1];
+      c += sred[3 * w + 2];
+    }
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+  }
+}
+
+__device__ __forceinline__ void block_min_key(unsigned long long k, unsigned long long* sred,
+                                              unsigned long long* slot) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(k, off);
+    k = o < k ? o : k;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sred[wave] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    unsigned long long m = sred[0];
+    for (int w = 1; w < nw; ++w) m = sred[w] < m ? sred[w] : m;
+    atomicMin(slot, m);
+  }
+}
+
+template <typename T>
+__host__ __device__ constexpr size_t stage_smem_bytes(int nv, bool visc, int bx) {
+  return (size_t)(3 * nv + (visc ? 9 : 0)) * bx * sizeof(T) + 64 * sizeof(double);
+}
+
+template <typename T, int RECON, int RS, int NV, bool VISC>
+__global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
+  if (a.ctrl->done) return;
+  extern __shared__ double smem_d[];
+  const int BX = blockDim.x;
+  const int t = threadIdx.x;
+  T* sW = reinterpret_cast<T*>(smem_d);   // [NV][BX] primitives of the row being updated
+  T* sH = sW + NV * BX;                   // [NV][BX] x high edges
+  T* sF = sH + NV * BX;                   // [NV][BX] x face fluxes
+  T* sV = sF + NV * BX;                   // [3 rows][rho, vx, vy][BX] (viscous only)
+  double* sred = reinterpret_cast<double*>(sF + (NV + (VISC ? 9 : 0)) * BX);
+
+  const int nx = a.nx, ny = a.ny;
+  const int OX = BX - 6;
+  const int x0 = blockIdx.x * OX;
+  int xc = (x0 + t - 3) % nx;
+  if (xc < 0) xc += nx;
+  const bool col_out = (t >= 3) && (t < BX - 3) && (x0 + t - 3 < nx);
+  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0);
+  const int tp1 = min(t + 1, BX - 1), tp2 = min(t + 2, BX - 1);
+  const int y0 = blockIdx.y * a.rows;
+  const int yend = min(y0 + a.rows, ny);
+  const int nit = (yend - y0) + 6;
+
+  const double dt = a.ctrl->dt;
+  const T sdt = T(a.beta * dt);
+  const Gas<T> gas{a.cs, a.cs2, a.inv_cs};
+  const Visc<T> vp{a.visc, a.smag_c, a.smag_on != 0};
+  const T dfloor = a.dfloor;
+  const T drag_coef = sdt * a.drag;
+
+  T w[5][NV];   // primitives of rows r-4 .. r in this column
+  T u[4][NV];   // floored conserved rows r-3 .. r
+  T hy[NV];     // y high edge of row r-3
+  T gp[NV];     // y flux at the low face of row r-3
+  T gn[NV];     // y flux at the low face of row r-2
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    hy[v] = gp[v] = gn[v] = T(0);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k][v] = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k][v] = T(0);
+  }
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  unsigned long long kmin = ~0ull;
+
+  for (int i = 0; i < nit; ++i) {
+    const int r = y0 - 3 + i;
+    int yr = r % ny;
+    if (yr < 0) yr += ny;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k][v] = w[k + 1][v];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) u[k][v] = u[k + 1][v];
+    }
+    {
+      const T* src = a.u_in + ((size_t)yr * nx + xc) * NV;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) u[3][v] = src[v];
+      // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
+      const T rho = (u[3][0] < dfloor) ? dfloor : u[3][0];
+      u[3][0] = rho;
+      const T inv = dvd(T(1), rho);
+      w[4][0] = rho;
+#pragma unroll
+      for (int v = 1; v < NV; ++v) w[4][v] = inv * u[3][v];
+    }
+    if constexpr (VISC) {
+      if (i >= 2) {
+        T* slot = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;
+        slot[t] = w[2][0];
+        slot[BX + t] = w[2][1];
+        slot[2 * BX + t] = w[2][2];
+      }
+    }
+    if (i >= 6) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sW[v * BX + t] = w[1][v];
+    }
+    if (VISC ? (i >= 2) : (i >= 6)) __syncthreads();   // B1
+
+    // ---- y direction: edges of row r-2, flux at its low face -------------------------
+    if (i >= 4) {
+      T lo[NV], hi[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        cell_edges<RECON>(w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], a.eps, lo[v], hi[v]);
+      if constexpr (floors_faces<RECON>()) {
+        lo[0] = fmx(lo[0], dfloor);
+        hi[0] = fmx(hi[0], dfloor);
+      }
+      if (i >= 5) {
+        // y faces: normal = vy (index 2), tangent = vx (index 1) (fluxes.py:209-214)
+        T f0, f1, f2;
+        riemann<RS>(hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
+        gn[0] = f0;
+        gn[1] = f2;
+        gn[2] = f1;
+#pragma unroll
+        for (int v = 3; v < NV; ++v) gn[v] = f0 * ((f0 >= T(0)) ? hy[v] : lo[v]);
+        if constexpr (VISC) {
+          // low y face of row s = r-2, column xc (fluxes.py:227-238 with axis=0)
+          const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row s
+          const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row s-1
+          const T vx_s = w[2][1], vy_s = w[2][2], vx_m = w[1][1], vy_m = w[1][2];
+          const T uy = dvd(vx_s - vx_m, a.dy);     // normal derivatives
+          const T vy = dvd(vy_s - vy_m, a.dy);
+          const T ux = dvd((rs[BX + tp1] + rm[BX + tp1]) - (rs[BX + tm1] + rm[BX + tm1]),
+                           a.four_dx);
+          const T vx = dvd((rs[2 * BX + tp1] + rm[2 * BX + tp1]) -
+                               (rs[2 * BX + tm1] + rm[2 * BX + tm1]), a.four_dx);
+          T fmx_, fmy_;
+          visc_face<0>(ux, vy, uy, vx, w[2][0] + w[1][0], vp, fmx_, fmy_);
+          gn[1] = gn[1] + fmx_;
+          gn[2] = gn[2] + fmy_;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) hy[v] = hi[v];
+    }
+
+    // ---- x direction for row j = r-3, then the update --------------------------------
+    if (i >= 6) {
+      T lo[NV], hi[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const T* row = sW + v * BX;
+        cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], a.eps, lo[v], hi[v]);
+      }
+      if constexpr (floors_faces<RECON>()) {
+        lo[0] = fmx(lo[0], dfloor);
+        hi[0] = fmx(hi[0], dfloor);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sH[v * BX + t] = hi[v];
+      __syncthreads();   // B2
+      T fx[NV];
+      {
+        T l[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) l[v] = sH[v * BX + tm1];
+        T f0, f1, f2;
+        riemann<RS>(l[0], lo[0], l[1], lo[1], l[2], lo[2], gas, f0, f1, f2);
+        fx[0] = f0;
+        fx[1] = f1;
+        fx[2] = f2;
+#pragma unroll
+        for (int v = 3; v < NV; ++v) fx[v] = f0 * ((f0 >= T(0)) ? l[v] : lo[v]);
+        if constexpr (VISC) {
+          // low x face of row j = r-3 at column xc (fluxes.py:227-238 with axis=1)
+          const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row j
+          const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row j+1
+          const T* rq = sV + ((r - 4 + 3 * ny + 3) % 3) * 3 * BX;   // row j-1
+          const T ux = dvd(rj[BX + t] - rj[BX + tm1], a.dx);
+          const T vx = dvd(rj[2 * BX + t] - rj[2 * BX + tm1], a.dx);
+          const T uy = dvd((rp[BX + t] + rp[BX + tm1]) - (rq[BX + t] + rq[BX + tm1]), a.four_dy);
+          const T vy = dvd((rp[2 * BX + t] + rp[2 * BX + tm1]) -
+                               (rq[2 * BX + t] + rq[2 * BX + tm1]), a.four_dy);
+          T fmx_, fmy_;
+          visc_face<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
+          fx[1] = fx[1] + fmx_;
+          fx[2] = fx[2] + fmy_;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sF[v * BX + t] = fx[v];
+      __syncthreads();   // B3
+      if (col_out) {
+        const int j = r - 3;
+        const size_t idx = ((size_t)j * nx + (x0 + t - 3)) * NV;
+        T un[NV];
+        T us[NV];
+        if (a.stage1) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) us[v] = u[0][v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) us[v] = a.u_sv[idx + v];
+          if (a.floor_saved) us[0] = (us[0] < dfloor) ? dfloor : us[0];
+        }
+        if (a.has_delta) {
+          // RK4 saved-register accumulation (hydro.py:42-43)
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            us[v] = us[v] + T(a.delta) * u[0][v];
+            a.s_out[idx + v] = us[v];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const T div = dvd(sF[v * BX + tp1] - fx[v], a.dx) + dvd(gn[v] - gp[v], a.dy);
+          if (a.blended)
+            un[v] = us[v] + (T(a.gam0) * (u[0][v] - us[v]) - sdt * div);
+          else
+            un[v] = (T(a.gam0) * u[0][v] + T(a.gam1) * us[v]) - sdt * div;
+        }
+        if (a.drag_on) {
+          // drag with the stage's incoming primitives (hydro.py:122-125)
+          const T c = drag_coef * w[1][0];
+          un[1] = un[1] + (-(c * w[1][1]));
+          un[2] = un[2] + (-(c * w[1][2]));
+        }
+        if (a.partial) {
+          // The kick normalises against the post-drag state; the floor comes after it.
+          const T* rf = a.raw + ((size_t)j * nx + (x0 + t - 3)) * 2;
+          acc0 += (double)un[0];
+          acc1 += (double)(un[0] * rf[0]);
+          acc2 += (double)(un[0] * rf[1]);
+        } else {
+          un[0] = (un[0] < dfloor) ? dfloor : un[0];
+          if (a.dtkey) {
+            const T inv = dvd(T(1), un[0]);
+            const T lx = dvd(a.dx, fabs(inv * un[1]) + a.cs);
+            const T ly = dvd(a.dy, fabs(inv * un[2]) + a.cs);
+            const unsigned long long kx = key_of((double)lx), ky = key_of((double)ly);
+            kmin = kx < kmin ? kx : kmin;
+            kmin = ky < kmin ? ky : kmin;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.u_out[idx + v] = un[v];
+      }
+    }
+    if (i >= 5) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) gp[v] = gn[v];
+    }
+  }
+
+  if (a.partial) {
+    block_reduce_store<T>(acc0, acc1, acc2, sred,
+                          a.partial + 3 * (size_t)(blockIdx.y * gridDim.x + blockIdx.x));
+  } else if (a.dtkey) {
+    block_min_key(kmin, reinterpret_cast<unsigned long long*>(sred), a.dtkey);
+  }
+}
+
+// Host-side launch for one (precision, reconstruction) pair; the Riemann solver, the
+// number of variables and the viscous switch are dispatched here.
+template <typename T, int RECON>
+hipError_t launch_stage(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 grid, int bx,
+                        hipStream_t st) {
+#define JX_LAUNCH(RS_, NV_, V_)                                                             \
+  do {                                                                                     \
+    hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_>), grid, dim3(bx),             \
+                       stage_smem_bytes<T>(NV_, V_, bx), st, a);                           \
+    return hipGetLastError();                                                              \
+  } while (0)
+#define JX_NV(RS_)                                                                          \
+  do {                                                                                     \
+    if (nv == 3) { if (visc) JX_LAUNCH(RS_, 3, true); else JX_LAUNCH(RS_, 3, false); }     \
+    if (nv == 4) { if (visc) JX_LAUNCH(RS_, 4, true); else JX_LAUNCH(RS_, 4, false); }     \
+    if (nv == 5) { if (visc) JX_LAUNCH(RS_, 5, true); else JX_LAUNCH(RS_, 5, false); }     \
+  } while (0)
+  if (rs == RS_LLF) JX_NV(RS_LLF);
+  if (rs == RS_HLLE) JX_NV(RS_HLLE);
+  if (rs == RS_ROE) JX_NV(RS_ROE);
+#undef JX_NV
+#undef JX_LAUNCH
+  return hipErrorInvalidValue;
+}
+
+}  // namespace jx
+
+// Each stage_<prec>_<recon>.hip translation unit instantiates one launcher.
+#define JX_STAGE_LAUNCHER(T, RECON, NAME)                                                   \
+  hipError_t NAME(int rs, int nv, bool visc, const jx::StageArgs<T>& a, dim3 grid, int bx,  \
+                  hipStream_t st) {                                                        \
+    return jx::launch_stage<T, RECON>(rs, nv, visc, a, grid, bx, st);                      \
+  }

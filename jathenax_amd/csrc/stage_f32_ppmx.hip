@@ -1,0 +1,4 @@
+// Fused stage kernels for float state with ppmx reconstruction (all solvers, nvar 3-5).
+#include "jx_stage.hpp"
+
+JX_STAGE_LAUNCHER(float, jx::RECON_PPMX, jx_stage_f32_ppmx)

@@ -1,0 +1,87 @@
+"""Device-resident time loop (jathenax/run.py:20-69) and the FP64 box diagnostics.
+
+`evolve_isothermal` keeps the reference signature.  The loop (recover, timestep, forcing
+refresh, step) runs on the GPU: each step is a begin-step control kernel, an optional
+render, the fused RK stages and, when forced, the kick; the host only polls the control
+block between chunks of steps.  `forcing` is ``(coefficients, basis)`` as built by the
+reference's CLI (run.py:148-155): coefficients (updates, modes, 2, 2) and the basis
+tuple of forcing.forcing_basis.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import as_device, context_for, dtype_id, ptr
+from .hydro import run_params
+
+
+def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
+                      viscosity=0.0, smagorinsky_cs=0.0, drag_rate=0.0,
+                      forcing=None, dt_update=1.0, tdriv_start=0.0, dedt=0.0,
+                      reconstruct="ppm4", rsolver="roe", predict_stress=None, sgs_weights=None):
+    """Advance to tlim or nlim cycles; returns (U, time, dt, cycle)."""
+    for name, value in (("viscosity", viscosity), ("smagorinsky_cs", smagorinsky_cs),
+                        ("drag_rate", drag_rate), ("dedt", dedt)):
+        if not math.isfinite(value) or value < 0.0:
+            raise ValueError(f"{name} must be finite and nonnegative")
+    if forcing is not None and (not 0.0 < dt_update < np.inf or not math.isfinite(tdriv_start)):
+        raise ValueError("dt_update must be finite and positive and tdriv_start finite")
+    if predict_stress is not None:
+        from .trace import evolve_with_predictor
+        return evolve_with_predictor(
+            U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, viscosity=viscosity,
+            smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
+            dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt, reconstruct=reconstruct,
+            rsolver=rsolver, predict_stress=predict_stress, sgs_weights=sgs_weights)
+    U = as_device(U)
+    p = run_params(U, dx, dy, cs, dfloor, integrator, reconstruct=reconstruct, rsolver=rsolver,
+                   viscosity=viscosity, smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate,
+                   cfl=cfl, tlim=tlim, nlim=min(int(nlim), 2 ** 31 - 1), dt_update=dt_update,
+                   tdriv_start=tdriv_start, dedt=dedt)
+    keep = []
+    fptr = None
+    if forcing is not None:
+        coefficients, basis = forcing
+        coef = as_device(coefficients, dtype=U.dtype, device=U.device)
+        tabs = [as_device(b, dtype=U.dtype, device=U.device) for b in basis]
+        if coef.ndim != 4 or tuple(coef.shape[2:]) != (2, 2):
+            raise ValueError("forcing coefficients must have shape (updates, modes, 2, 2)")
+        nm = coef.shape[1]
+        if (tuple(tabs[0].shape) != (nm, U.shape[1]) or tuple(tabs[2].shape) != (nm, U.shape[0])):
+            raise ValueError("forcing basis does not match the modes and grid")
+        keep = [coef, *tabs]
+        f = _lib.Forcing(coefficients=coef.data_ptr(), cos_x=tabs[0].data_ptr(),
+                         sin_x=tabs[1].data_ptr(), cos_y=tabs[2].data_ptr(),
+                         sin_y=tabs[3].data_ptr(), n_updates=coef.shape[0], n_modes=nm)
+        fptr = ctypes.byref(f)
+    out = torch.empty_like(U)
+    t, dt, cyc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    ctx = context_for(U)
+    ctx.check(_lib.load().jx_evolve_isothermal(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
+                                               fptr, ctypes.byref(t), ctypes.byref(dt),
+                                               ctypes.byref(cyc)), "evolve_isothermal")
+    del keep
+    return out, t.value, dt.value, cyc.value
+
+
+def conserved_sums(U):
+    """Float64 sums of each conserved component over the box (run.py:186-187 / (dx dy))."""
+    U = as_device(U)
+    ny, nx, nv = U.shape
+    out = (ctypes.c_double * nv)()
+    ctx = context_for(U)
+    ctx.check(_lib.load().jx_conserved_sums(ctx.handle, dtype_id(U), ny, nx, nv, ptr(U), out),
+              "conserved_sums")
+    return np.asarray(out[:], dtype=np.float64)
+
+
+def check_finished(U, time, cycle, tlim, nlim):
+    """The reference CLI's end-of-run check (run.py:178-180)."""
+    reached = time == tlim
+    finite = bool(torch.isfinite(U).all()) and math.isfinite(time)
+    if not finite or (not reached and cycle != nlim):
+        raise RuntimeError(f"Periodic run ended at t={time} after {cycle} steps")
