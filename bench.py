@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=2048, help="CPU baseline sample grid size")
     ap.add_argument("--block-x", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 512 MiB device-to-device copy (PMC byte calibration)")
     return ap.parse_args()
 
 
@@ -104,6 +106,12 @@ def main():
         sys.exit(1)
     n, K = a.n, a.steps
     nvar, stages = 3, 3
+    if a.calibrate_copy:
+        src = torch.ones(128 * 2 ** 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(f"# bench: {n}x{n} {a.dtype} steps={K} warmup={a.warmup}", file=sys.stderr, flush=True)
     main_leg = gpu_leg(jx, problems, n, a.dtype, K, a.warmup, a.block_x, a.rows)
     zu = n * n * stages * K / main_leg["wall"]

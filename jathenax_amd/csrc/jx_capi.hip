@@ -19,7 +19,8 @@ using jx::StageArgs;
 // Stage launchers, one per (precision, reconstruction) translation unit.
 #define JX_DECL(T, NAME)                                                                    \
   hipError_t NAME(int rs, int nv, bool visc, const jx::StageArgs<T>& a, dim3 grid, int bx, \
-                  hipStream_t st);
+                  hipStream_t st);                                                         \
+  int NAME##_occupancy(int rs, int nv, bool visc, int bx, bool exact);
 JX_DECL(double, jx_stage_f64_dc)
 JX_DECL(double, jx_stage_f64_plm)
 JX_DECL(double, jx_stage_f64_ppm4)
@@ -46,6 +47,7 @@ struct jx_ctx {
   double* sums = nullptr;     // small device result vector
   double* host_sums = nullptr;
   int block_x = 0, rows = 0;
+  int cus = 0;                // compute units of the device
   std::string err;
   // Optional HIP-event timing of the fused stage launches.
   bool timing = false;
@@ -144,6 +146,15 @@ int check_params(jx_ctx* c, const jx_run_params* p) {
 
 template <typename T> struct StageLaunch;
 template <> struct StageLaunch<double> {
+  static int occupancy(int recon, int rs, int nv, bool visc, int bx, bool exact) {
+    switch (recon) {
+      case jx::RECON_DC: return jx_stage_f64_dc_occupancy(rs, nv, visc, bx, exact);
+      case jx::RECON_PLM: return jx_stage_f64_plm_occupancy(rs, nv, visc, bx, exact);
+      case jx::RECON_PPM4: return jx_stage_f64_ppm4_occupancy(rs, nv, visc, bx, exact);
+      case jx::RECON_PPMX: return jx_stage_f64_ppmx_occupancy(rs, nv, visc, bx, exact);
+      default: return jx_stage_f64_wenoz_occupancy(rs, nv, visc, bx, exact);
+    }
+  }
   static hipError_t go(int recon, int rs, int nv, bool visc, const StageArgs<double>& a,
                        dim3 g, int bx, hipStream_t st) {
     switch (recon) {
@@ -156,6 +167,15 @@ template <> struct StageLaunch<double> {
   }
 };
 template <> struct StageLaunch<float> {
+  static int occupancy(int recon, int rs, int nv, bool visc, int bx, bool exact) {
+    switch (recon) {
+      case jx::RECON_DC: return jx_stage_f32_dc_occupancy(rs, nv, visc, bx, exact);
+      case jx::RECON_PLM: return jx_stage_f32_plm_occupancy(rs, nv, visc, bx, exact);
+      case jx::RECON_PPM4: return jx_stage_f32_ppm4_occupancy(rs, nv, visc, bx, exact);
+      case jx::RECON_PPMX: return jx_stage_f32_ppmx_occupancy(rs, nv, visc, bx, exact);
+      default: return jx_stage_f32_wenoz_occupancy(rs, nv, visc, bx, exact);
+    }
+  }
   static hipError_t go(int recon, int rs, int nv, bool visc, const StageArgs<float>& a,
                        dim3 g, int bx, hipStream_t st) {
     switch (recon) {
@@ -168,13 +188,31 @@ template <> struct StageLaunch<float> {
   }
 };
 
-void pick_tiles(const jx_ctx* c, int nx, int ny, int* bx, int* rows) {
+bool is_pow2(double x) {
+  if (!(x > 0.0) || !std::isfinite(x)) return false;
+  int e = 0;
+  return std::frexp(x, &e) == 0.5 && x >= 1e-300 && x <= 1e300;
+}
+
+// Strip width: the whole row when it fits one block, else 256 threads.  Rows per block:
+// given, or sized so that the grid is one full wave of resident blocks (no tail wave);
+// at least 32 rows so the 6-row warm-up of each block stays small.
+template <typename T>
+void pick_tiles(const jx_ctx* c, const jx_run_params* p, bool visc, bool exact, int* bx,
+                int* rows) {
   int b = c->block_x;
-  if (b <= 0) b = std::min(jx::kMaxBlock, 64 * ((nx + 6 + 63) / 64));
+  if (b <= 0) b = std::min(jx::kMaxBlock, 64 * ((p->nx + 6 + 63) / 64));
   int r = c->rows;
-  if (r <= 0) r = 64;
+  if (r <= 0) {
+    const int strips = (p->nx + (b - 6) - 1) / (b - 6);
+    const int per_cu = StageLaunch<T>::occupancy(p->reconstruct, p->rsolver, p->nvar, visc, b,
+                                                 exact);
+    const long long slots = (long long)std::max(per_cu, 1) * std::max(c->cus, 1);
+    const long long per_strip = std::max<long long>(1, slots / strips);
+    r = (int)std::max<long long>(32, (p->ny + per_strip - 1) / per_strip);
+  }
   *bx = b;
-  *rows = std::max(1, std::min(r, ny));
+  *rows = std::max(1, std::min(r, p->ny));
 }
 
 // Everything the stage launches of one step need.
@@ -190,7 +228,8 @@ template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p) {
   StepPlan<T> s;
   s.p = p;
   s.visc = p->viscosity > 0.0 || p->smagorinsky_cs > 0.0;
-  pick_tiles(c, p->nx, p->ny, &s.bx, &s.rows);
+  const bool exact = is_pow2(p->dx) && is_pow2(p->dy) && is_pow2(p->cs);
+  pick_tiles<T>(c, p, s.visc, exact, &s.bx, &s.rows);
   s.grid = dim3((unsigned)((p->nx + (s.bx - 6) - 1) / (s.bx - 6)),
                 (unsigned)((p->ny + s.rows - 1) / s.rows));
   StageArgs<T> a{};
@@ -207,6 +246,11 @@ template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p) {
   a.eps = T(std::max(1e-42, (double)std::numeric_limits<T>::min()));
   a.four_dx = T(4.0 * p->dx);
   a.four_dy = T(4.0 * p->dy);
+  a.inv_dx = T(1.0 / p->dx);
+  a.inv_dy = T(1.0 / p->dy);
+  a.inv_four_dx = T(1.0 / (4.0 * p->dx));
+  a.inv_four_dy = T(1.0 / (4.0 * p->dy));
+  a.exact = exact;
   a.visc = T(p->viscosity);
   a.smag_on = p->smagorinsky_cs > 0.0;
   a.smag_c = T(p->smagorinsky_cs * p->smagorinsky_cs * p->dx * p->dy);
@@ -501,6 +545,8 @@ int jx_ctx_create(jx_ctx** out, int device, void* stream) {
   c->device = device;
   c->stream = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipMalloc(&c->ctrl, sizeof(Ctrl));
   if (e == hipSuccess) e = hipHostMalloc(&c->host_ctrl, sizeof(Ctrl), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&c->sums, 16 * sizeof(double));

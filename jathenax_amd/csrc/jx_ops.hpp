@@ -30,9 +30,46 @@ template <typename T> __device__ __forceinline__ T divk(T x, double c) {
 }
 template <typename T> __device__ __forceinline__ T sqr(T x) { return x * x; }
 template <typename T> __device__ __forceinline__ T root(T x) { return sqrt(x); }
-template <> __device__ __forceinline__ float root<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ float root<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
 template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return fmin(a, b); }
 template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return fmax(a, b); }
+// fmin(fmax(e, fmin(a, b)), fmax(a, b)): clamp e between a and b (one v_med3 in FP32).
+template <typename T> __device__ __forceinline__ T clamp_between(T e, T a, T b) {
+  if constexpr (Fast<T>::value) return __builtin_amdgcn_fmed3f(e, a, b);
+  else return fmn(fmx(e, fmn(a, b)), fmx(a, b));
+}
+
+// Division by a run constant d (dx, dy, cs, 4dx, 4dy).  EXACT: d is a power of two, so
+// x / d == x * (1/d) bit for bit and the multiply is used even in FP64.
+template <typename T, bool EXACT> struct Over {
+  T d, inv;
+  __device__ __forceinline__ T operator()(T x) const {
+    if constexpr (Fast<T>::value || EXACT) return x * inv;
+    else return x / d;
+  }
+};
+
+// PPM4 interface value between cells c-1 and c, clamped between them (fluxes.py:29-32).
+// The reference evaluates it twice, as the low edge of c and the high edge of c-1; the
+// operand order of both evaluations is the same up to commuted additions, so one value
+// serves both cells.
+template <typename T>
+__device__ __forceinline__ T ppm4_interface(T qm2, T qm1, T q, T qp1) {
+  return clamp_between(divk(T(7) * (q + qm1) - (qm2 + qp1), 12.0), q, qm1);
+}
+
+// PPM4 extremum and asymmetric-parabola limiter on clamped edges (fluxes.py:34-41).
+template <typename T>
+__device__ __forceinline__ void ppm4_limit(T q, T& lo, T& hi) {
+  const T dh = hi - q, dl = lo - q;
+  const bool ext = dh * dl >= T(0);
+  const T l2 = (fabs(dl) >= T(2) * fabs(dh)) ? q - T(2) * dh : lo;
+  const T h2 = (fabs(dh) >= T(2) * fabs(dl)) ? q - T(2) * dl : hi;
+  lo = ext ? q : l2;
+  hi = ext ? q : h2;
+}
 
 // ---------------------------------------------------------------------------------------
 // Reconstruction: low and high edge values of one cell from its 5-point stencil.  The
@@ -68,16 +105,9 @@ __device__ __forceinline__ void cell_edges(T qm2, T qm1, T q, T qp1, T qp2, T ep
     lo = q - s;
   } else if constexpr (RECON == RECON_PPM4) {
     // fluxes.py:21-42
-    T l = divk(T(7) * (q + qm1) - (qm2 + qp1), 12.0);
-    T h = divk(T(7) * (q + qp1) - (qm1 + qp2), 12.0);
-    l = fmn(fmx(l, fmn(q, qm1)), fmx(q, qm1));
-    h = fmn(fmx(h, fmn(q, qp1)), fmx(q, qp1));
-    const T dh = h - q, dl = l - q;
-    const bool ext = dh * dl >= T(0);
-    const T l2 = (fabs(dl) >= T(2) * fabs(dh)) ? q - T(2) * dh : l;
-    const T h2 = (fabs(dh) >= T(2) * fabs(dl)) ? q - T(2) * dl : h;
-    lo = ext ? q : l2;
-    hi = ext ? q : h2;
+    lo = ppm4_interface(qm2, qm1, q, qp1);
+    hi = ppm4_interface(qm1, q, qp1, qp2);
+    ppm4_limit(q, lo, hi);
   } else if constexpr (RECON == RECON_PPMX) {
     // fluxes.py:56-84
     T l = divk(T(7) * (q + qm1) - (qm2 + qp1), 12.0);
@@ -147,7 +177,7 @@ template <typename T> struct Gas {
   T cs, cs2, inv_cs;
 };
 
-template <int RS, typename T>
+template <int RS, bool EXACT, typename T>
 __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
                                         T& f0, T& f1, T& f2) {
   const T cs = g.cs, cs2 = g.cs2;
@@ -191,7 +221,7 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     const T wm = vn - cs, wp = vn + cs;
     const T drho = rr - rl, dmn = mnr - mnl, dmt = mtr - mtl;
     T am, ap;
-    if constexpr (Fast<T>::value) {
+    if constexpr (Fast<T>::value || EXACT) {
       am = drho * (T(0.5) + T(0.5) * vn * g.inv_cs) - dmn * T(0.5) * g.inv_cs;
       ap = drho * (T(0.5) - T(0.5) * vn * g.inv_cs) + dmn * T(0.5) * g.inv_cs;
     } else {

@@ -16,6 +16,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "jx_ops.hpp"
 
 namespace jx {
@@ -35,7 +37,9 @@ template <typename T> struct StageArgs {
   int stage1;                   // saved register is the (floored) stage input itself
   int floor_saved;              // apply the density floor when reading u_sv
   int has_delta, blended, drag_on, smag_on;
+  int exact;                    // dx, dy, cs are powers of two
   T dx, dy, cs, cs2, inv_cs, dfloor, eps, four_dx, four_dy, visc, smag_c, drag;
+  T inv_dx, inv_dy, inv_four_dx, inv_four_dy;
   double gam0, gam1, beta, delta;
 };
 
@@ -91,9 +95,17 @@ __host__ __device__ constexpr size_t stage_smem_bytes(int nv, bool visc, int bx)
   return (size_t)(3 * nv + (visc ? 9 : 0)) * bx * sizeof(T) + 64 * sizeof(double);
 }
 
-template <typename T, int RECON, int RS, int NV, bool VISC>
-__global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
+template <int P> using IC = std::integral_constant<int, P>;
+
+template <typename T, int RECON, int RS, int NV, bool VISC, bool EXACT>
+__global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
+  // A local copy keeps the lambda below from taking the kernel argument's address.
+  const StageArgs<T> a = args;
   if (a.ctrl->done) return;
+  // PPM4 shares each y interface between the two cells it bounds, so its register ring
+  // holds rows r-3..r; the other reconstructions need the full 5-row stencil.
+  constexpr bool SHARED = RECON == RECON_PPM4;
+  constexpr int R = SHARED ? 4 : 5;
   extern __shared__ double smem_d[];
   const int BX = blockDim.x;
   const int t = threadIdx.x;
@@ -119,76 +131,90 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
   const T sdt = T(a.beta * dt);
   const Gas<T> gas{a.cs, a.cs2, a.inv_cs};
   const Visc<T> vp{a.visc, a.smag_c, a.smag_on != 0};
+  const Over<T, EXACT> over_dx{a.dx, a.inv_dx}, over_dy{a.dy, a.inv_dy};
+  const Over<T, EXACT> over_4dx{a.four_dx, a.inv_four_dx}, over_4dy{a.four_dy, a.inv_four_dy};
   const T dfloor = a.dfloor;
   const T drag_coef = sdt * a.drag;
 
-  T w[5][NV];   // primitives of rows r-4 .. r in this column
-  T u[4][NV];   // floored conserved rows r-3 .. r
+  T w[R][NV];   // primitives, row q in slot (q - y0 + 3) mod R
+  T u[R][NV];   // floored conserved state, same slots
+  T ip[NV];     // PPM4: clamped interface below row r-2
   T hy[NV];     // y high edge of row r-3
   T gp[NV];     // y flux at the low face of row r-3
   T gn[NV];     // y flux at the low face of row r-2
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    hy[v] = gp[v] = gn[v] = T(0);
+    ip[v] = hy[v] = gp[v] = gn[v] = T(0);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) w[k][v] = T(0);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) u[k][v] = T(0);
+    for (int k = 0; k < R; ++k) w[k][v] = u[k][v] = T(0);
   }
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
   unsigned long long kmin = ~0ull;
+  T pf[NV];     // prefetched conserved values of the next row
+  {
+    int yn = (y0 - 3) % ny;
+    if (yn < 0) yn += ny;
+    const T* src = a.u_in + ((size_t)yn * nx + xc) * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) pf[v] = src[v];
+  }
 
-  for (int i = 0; i < nit; ++i) {
+  auto body = [&](auto pc, int i) {
+    constexpr int P = decltype(pc)::value;
+    constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
+    constexpr int S4 = (P + R - 4) % R;   // row r-4 (5-row ring only)
     const int r = y0 - 3 + i;
-    int yr = r % ny;
-    if (yr < 0) yr += ny;
+    {
+      // Row r was fetched one iteration ago; issue row r+1 now so its latency overlaps
+      // this row's arithmetic.
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+      for (int v = 0; v < NV; ++v) u[S0][v] = pf[v];
+      if (i + 1 < nit) {
+        int yn = (r + 1) % ny;
+        if (yn < 0) yn += ny;
+        const T* src = a.u_in + ((size_t)yn * nx + xc) * NV;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) w[k][v] = w[k + 1][v];
+        for (int v = 0; v < NV; ++v) pf[v] = src[v];
+      }
+    }
+    // Saved register of the row updated this iteration (pointwise, stages >= 2).
+    T svr[NV];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) u[k][v] = u[k + 1][v];
+    for (int v = 0; v < NV; ++v) svr[v] = T(0);
+    if (i >= 6 && !a.stage1) {
+      const T* sv = a.u_sv + ((size_t)(r - 3) * nx + xc) * NV;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) svr[v] = sv[v];
     }
     {
-      const T* src = a.u_in + ((size_t)yr * nx + xc) * NV;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) u[3][v] = src[v];
       // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
-      const T rho = (u[3][0] < dfloor) ? dfloor : u[3][0];
-      u[3][0] = rho;
+      const T rho = (u[S0][0] < dfloor) ? dfloor : u[S0][0];
+      u[S0][0] = rho;
       const T inv = dvd(T(1), rho);
-      w[4][0] = rho;
+      w[S0][0] = rho;
 #pragma unroll
-      for (int v = 1; v < NV; ++v) w[4][v] = inv * u[3][v];
+      for (int v = 1; v < NV; ++v) w[S0][v] = inv * u[S0][v];
     }
     if constexpr (VISC) {
       if (i >= 2) {
         T* slot = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;
-        slot[t] = w[2][0];
-        slot[BX + t] = w[2][1];
-        slot[2 * BX + t] = w[2][2];
+        slot[t] = w[S2][0];
+        slot[BX + t] = w[S2][1];
+        slot[2 * BX + t] = w[S2][2];
       }
     }
     if (i >= 6) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sW[v * BX + t] = w[1][v];
+      for (int v = 0; v < NV; ++v) sW[v * BX + t] = w[S3][v];
     }
     if (VISC ? (i >= 2) : (i >= 6)) __syncthreads();   // B1
 
     // ---- y direction: edges of row r-2, flux at its low face -------------------------
-    if (i >= 4) {
-      T lo[NV], hi[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-        cell_edges<RECON>(w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], a.eps, lo[v], hi[v]);
-      if constexpr (floors_faces<RECON>()) {
-        lo[0] = fmx(lo[0], dfloor);
-        hi[0] = fmx(hi[0], dfloor);
-      }
+    auto y_face = [&](T (&lo)[NV], T (&hi)[NV]) {
       if (i >= 5) {
         // y faces: normal = vy (index 2), tangent = vx (index 1) (fluxes.py:209-214)
         T f0, f1, f2;
-        riemann<RS>(hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
+        riemann<RS, EXACT>(hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
         gn[0] = f0;
         gn[1] = f2;
         gn[2] = f1;
@@ -198,60 +224,91 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
           // low y face of row s = r-2, column xc (fluxes.py:227-238 with axis=0)
           const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row s
           const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row s-1
-          const T vx_s = w[2][1], vy_s = w[2][2], vx_m = w[1][1], vy_m = w[1][2];
-          const T uy = dvd(vx_s - vx_m, a.dy);     // normal derivatives
-          const T vy = dvd(vy_s - vy_m, a.dy);
-          const T ux = dvd((rs[BX + tp1] + rm[BX + tp1]) - (rs[BX + tm1] + rm[BX + tm1]),
-                           a.four_dx);
-          const T vx = dvd((rs[2 * BX + tp1] + rm[2 * BX + tp1]) -
-                               (rs[2 * BX + tm1] + rm[2 * BX + tm1]), a.four_dx);
+          const T uy = over_dy(w[S2][1] - w[S3][1]);     // normal derivatives
+          const T vy = over_dy(w[S2][2] - w[S3][2]);
+          const T ux = over_4dx((rs[BX + tp1] + rm[BX + tp1]) - (rs[BX + tm1] + rm[BX + tm1]));
+          const T vx = over_4dx((rs[2 * BX + tp1] + rm[2 * BX + tp1]) -
+                                (rs[2 * BX + tm1] + rm[2 * BX + tm1]));
           T fmx_, fmy_;
-          visc_face<0>(ux, vy, uy, vx, w[2][0] + w[1][0], vp, fmx_, fmy_);
+          visc_face<0>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
           gn[1] = gn[1] + fmx_;
           gn[2] = gn[2] + fmy_;
         }
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) hy[v] = hi[v];
+    };
+    if constexpr (SHARED) {
+      if (i >= 3) {
+        T in[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          in[v] = ppm4_interface(w[S3][v], w[S2][v], w[S1][v], w[S0][v]);   // below row r-1
+        if (i >= 4) {
+          T lo[NV], hi[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            lo[v] = ip[v];
+            hi[v] = in[v];
+            ppm4_limit(w[S2][v], lo[v], hi[v]);
+          }
+          y_face(lo, hi);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ip[v] = in[v];
+      }
+    } else {
+      if (i >= 4) {
+        T lo[NV], hi[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          cell_edges<RECON>(w[S4][v], w[S3][v], w[S2][v], w[S1][v], w[S0][v], a.eps, lo[v],
+                            hi[v]);
+        if constexpr (floors_faces<RECON>()) {
+          lo[0] = fmx(lo[0], dfloor);
+          hi[0] = fmx(hi[0], dfloor);
+        }
+        y_face(lo, hi);
+      }
     }
 
     // ---- x direction for row j = r-3, then the update --------------------------------
     if (i >= 6) {
-      T lo[NV], hi[NV];
+      T xl[NV] = {}, xh[NV] = {};
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const T* row = sW + v * BX;
-        cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], a.eps, lo[v], hi[v]);
+        cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], a.eps, xl[v], xh[v]);
       }
       if constexpr (floors_faces<RECON>()) {
-        lo[0] = fmx(lo[0], dfloor);
-        hi[0] = fmx(hi[0], dfloor);
+        xl[0] = fmx(xl[0], dfloor);
+        xh[0] = fmx(xh[0], dfloor);
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sH[v * BX + t] = hi[v];
+      for (int v = 0; v < NV; ++v) sH[v * BX + t] = xh[v];
       __syncthreads();   // B2
-      T fx[NV];
+      T fx[NV] = {};
       {
-        T l[NV];
+        T l[NV] = {};
 #pragma unroll
         for (int v = 0; v < NV; ++v) l[v] = sH[v * BX + tm1];
         T f0, f1, f2;
-        riemann<RS>(l[0], lo[0], l[1], lo[1], l[2], lo[2], gas, f0, f1, f2);
+        riemann<RS, EXACT>(l[0], xl[0], l[1], xl[1], l[2], xl[2], gas, f0, f1, f2);
         fx[0] = f0;
         fx[1] = f1;
         fx[2] = f2;
 #pragma unroll
-        for (int v = 3; v < NV; ++v) fx[v] = f0 * ((f0 >= T(0)) ? l[v] : lo[v]);
+        for (int v = 3; v < NV; ++v) fx[v] = f0 * ((f0 >= T(0)) ? l[v] : xl[v]);
         if constexpr (VISC) {
           // low x face of row j = r-3 at column xc (fluxes.py:227-238 with axis=1)
           const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row j
           const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row j+1
           const T* rq = sV + ((r - 4 + 3 * ny + 3) % 3) * 3 * BX;   // row j-1
-          const T ux = dvd(rj[BX + t] - rj[BX + tm1], a.dx);
-          const T vx = dvd(rj[2 * BX + t] - rj[2 * BX + tm1], a.dx);
-          const T uy = dvd((rp[BX + t] + rp[BX + tm1]) - (rq[BX + t] + rq[BX + tm1]), a.four_dy);
-          const T vy = dvd((rp[2 * BX + t] + rp[2 * BX + tm1]) -
-                               (rq[2 * BX + t] + rq[2 * BX + tm1]), a.four_dy);
+          const T ux = over_dx(rj[BX + t] - rj[BX + tm1]);
+          const T vx = over_dx(rj[2 * BX + t] - rj[2 * BX + tm1]);
+          const T uy = over_4dy((rp[BX + t] + rp[BX + tm1]) - (rq[BX + t] + rq[BX + tm1]));
+          const T vy = over_4dy((rp[2 * BX + t] + rp[2 * BX + tm1]) -
+                                (rq[2 * BX + t] + rq[2 * BX + tm1]));
           T fmx_, fmy_;
           visc_face<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
           fx[1] = fx[1] + fmx_;
@@ -264,37 +321,37 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
       if (col_out) {
         const int j = r - 3;
         const size_t idx = ((size_t)j * nx + (x0 + t - 3)) * NV;
-        T un[NV];
-        T us[NV];
+        T un[NV] = {};
+        T us[NV] = {};
         if (a.stage1) {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) us[v] = u[0][v];
+          for (int v = 0; v < NV; ++v) us[v] = u[S3][v];
         } else {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) us[v] = a.u_sv[idx + v];
+          for (int v = 0; v < NV; ++v) us[v] = svr[v];
           if (a.floor_saved) us[0] = (us[0] < dfloor) ? dfloor : us[0];
         }
         if (a.has_delta) {
           // RK4 saved-register accumulation (hydro.py:42-43)
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
-            us[v] = us[v] + T(a.delta) * u[0][v];
+            us[v] = us[v] + T(a.delta) * u[S3][v];
             a.s_out[idx + v] = us[v];
           }
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          const T div = dvd(sF[v * BX + tp1] - fx[v], a.dx) + dvd(gn[v] - gp[v], a.dy);
+          const T div = over_dx(sF[v * BX + tp1] - fx[v]) + over_dy(gn[v] - gp[v]);
           if (a.blended)
-            un[v] = us[v] + (T(a.gam0) * (u[0][v] - us[v]) - sdt * div);
+            un[v] = us[v] + (T(a.gam0) * (u[S3][v] - us[v]) - sdt * div);
           else
-            un[v] = (T(a.gam0) * u[0][v] + T(a.gam1) * us[v]) - sdt * div;
+            un[v] = (T(a.gam0) * u[S3][v] + T(a.gam1) * us[v]) - sdt * div;
         }
         if (a.drag_on) {
           // drag with the stage's incoming primitives (hydro.py:122-125)
-          const T c = drag_coef * w[1][0];
-          un[1] = un[1] + (-(c * w[1][1]));
-          un[2] = un[2] + (-(c * w[1][2]));
+          const T c = drag_coef * w[S3][0];
+          un[1] = un[1] + (-(c * w[S3][1]));
+          un[2] = un[2] + (-(c * w[S3][2]));
         }
         if (a.partial) {
           // The kick normalises against the post-drag state; the floor comes after it.
@@ -306,8 +363,8 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
           un[0] = (un[0] < dfloor) ? dfloor : un[0];
           if (a.dtkey) {
             const T inv = dvd(T(1), un[0]);
-            const T lx = dvd(a.dx, fabs(inv * un[1]) + a.cs);
-            const T ly = dvd(a.dy, fabs(inv * un[2]) + a.cs);
+            const T lx = dvd<T>(a.dx, T(fabs(inv * un[1])) + a.cs);
+            const T ly = dvd<T>(a.dy, T(fabs(inv * un[2])) + a.cs);
             const unsigned long long kx = key_of((double)lx), ky = key_of((double)ly);
             kmin = kx < kmin ? kx : kmin;
             kmin = ky < kmin ? ky : kmin;
@@ -321,6 +378,16 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) gp[v] = gn[v];
     }
+  };
+
+  for (int i0 = 0; i0 < nit; i0 += R) {
+    body(IC<0>{}, i0);
+    if (i0 + 1 < nit) body(IC<1>{}, i0 + 1);
+    if (i0 + 2 < nit) body(IC<2>{}, i0 + 2);
+    if (i0 + 3 < nit) body(IC<3>{}, i0 + 3);
+    if constexpr (R == 5) {
+      if (i0 + 4 < nit) body(IC<4 % R>{}, i0 + 4);
+    }
   }
 
   if (a.partial) {
@@ -333,12 +400,12 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> a) {
 
 // Host-side launch for one (precision, reconstruction) pair; the Riemann solver, the
 // number of variables and the viscous switch are dispatched here.
-template <typename T, int RECON>
-hipError_t launch_stage(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 grid, int bx,
-                        hipStream_t st) {
+template <typename T, int RECON, bool EXACT>
+hipError_t launch_stage_e(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 grid, int bx,
+                          hipStream_t st) {
 #define JX_LAUNCH(RS_, NV_, V_)                                                             \
   do {                                                                                     \
-    hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_>), grid, dim3(bx),             \
+    hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_, EXACT>), grid, dim3(bx),      \
                        stage_smem_bytes<T>(NV_, V_, bx), st, a);                           \
     return hipGetLastError();                                                              \
   } while (0)
@@ -356,11 +423,64 @@ hipError_t launch_stage(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 g
   return hipErrorInvalidValue;
 }
 
+// Resident blocks per CU for the kernel a launch would use (0 on error).
+template <typename T, int RECON, bool EXACT>
+int occupancy_e(int rs, int nv, bool visc, int bx) {
+  int nb = 0;
+#define JX_OCC(RS_, NV_, V_)                                                                \
+  do {                                                                                     \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                       \
+            &nb, stage_kernel<T, RECON, RS_, NV_, V_, EXACT>, bx,                          \
+            stage_smem_bytes<T>(NV_, V_, bx)) != hipSuccess)                               \
+      nb = 0;                                                                              \
+    return nb;                                                                             \
+  } while (0)
+#define JX_ONV(RS_)                                                                         \
+  do {                                                                                     \
+    if (nv == 3) { if (visc) JX_OCC(RS_, 3, true); else JX_OCC(RS_, 3, false); }           \
+    if (nv == 4) { if (visc) JX_OCC(RS_, 4, true); else JX_OCC(RS_, 4, false); }           \
+    if (nv == 5) { if (visc) JX_OCC(RS_, 5, true); else JX_OCC(RS_, 5, false); }           \
+  } while (0)
+  if (rs == RS_LLF) JX_ONV(RS_LLF);
+  if (rs == RS_HLLE) JX_ONV(RS_HLLE);
+  if (rs == RS_ROE) JX_ONV(RS_ROE);
+#undef JX_ONV
+#undef JX_OCC
+  return 0;
+}
+
+template <typename T, int RECON>
+int stage_occupancy(int rs, int nv, bool visc, int bx, bool exact) {
+  if constexpr (Fast<T>::value) {
+    return occupancy_e<T, RECON, true>(rs, nv, visc, bx);
+  } else {
+    return exact ? occupancy_e<T, RECON, true>(rs, nv, visc, bx)
+                 : occupancy_e<T, RECON, false>(rs, nv, visc, bx);
+  }
+}
+
+// FP32 always multiplies by reciprocals; FP64 does so only when dx, dy and cs are powers
+// of two (a.exact), where the product equals the IEEE quotient.
+template <typename T, int RECON>
+hipError_t launch_stage(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 grid, int bx,
+                        hipStream_t st) {
+  if constexpr (Fast<T>::value) {
+    return launch_stage_e<T, RECON, true>(rs, nv, visc, a, grid, bx, st);
+  } else {
+    if (a.exact) return launch_stage_e<T, RECON, true>(rs, nv, visc, a, grid, bx, st);
+    return launch_stage_e<T, RECON, false>(rs, nv, visc, a, grid, bx, st);
+  }
+}
+
 }  // namespace jx
 
-// Each stage_<prec>_<recon>.hip translation unit instantiates one launcher.
+// Each stage_<prec>_<recon>.hip translation unit instantiates one launcher and its
+// occupancy query.
 #define JX_STAGE_LAUNCHER(T, RECON, NAME)                                                   \
   hipError_t NAME(int rs, int nv, bool visc, const jx::StageArgs<T>& a, dim3 grid, int bx,  \
                   hipStream_t st) {                                                        \
     return jx::launch_stage<T, RECON>(rs, nv, visc, a, grid, bx, st);                      \
+  }                                                                                        \
+  int NAME##_occupancy(int rs, int nv, bool visc, int bx, bool exact) {                    \
+    return jx::stage_occupancy<T, RECON>(rs, nv, visc, bx, exact);                         \
   }
