@@ -135,6 +135,37 @@ int jx_modal_coefficients(const int* modes, int n_modes, double lx, double ly, i
                           int nhigh, double npeak, double sol_fraction, long long rseed,
                           double tcorr, double dt_update, int updates, double* out);
 
+/* ---- Batched ensemble engine -------------------------------------------------------
+ * No counterpart in the reference (which runs one grid per call): many independent
+ * instances of evolve_isothermal (jathenax/run.py:24-69), each with its own shape,
+ * operators, integrator, physics and forcing, advanced together by persistent kernels.
+ * One round = one RK stage of every active instance; rounds are replayed from a
+ * hipGraph.  Instances live in one HBM pool of pool_bytes; nvar is 3 or 4.  The batch
+ * owns its own HIP stream: jx_batch_add copies U0 (a device array ready on the caller's
+ * stream) into the pool, so the caller may free U0 afterwards.  Results are bitwise
+ * independent of which other instances share the batch. */
+typedef struct jx_batch jx_batch;
+int jx_batch_create(int device, int dtype, int max_instances, long long max_tiles,
+                    long long pool_bytes, jx_batch** out);
+void jx_batch_destroy(jx_batch* b);
+const char* jx_batch_last_error(const jx_batch* b);
+/* HBM bytes an instance of these parameters takes from the pool (admission control). */
+long long jx_batch_footprint(const jx_batch* b, const jx_run_params* p, int forced);
+long long jx_batch_pool_free(const jx_batch* b);
+/* Register an instance (tlim, nlim, cfl and the forcing schedule come from p); `rows`
+ * is its tile height (0 = default).  forcing may be NULL. */
+int jx_batch_add(jx_batch* b, const jx_run_params* p, const void* U0, const jx_forcing* f,
+                 int rows, int* id_out);
+int jx_batch_remove(jx_batch* b, int id);
+/* Run at least `rounds` rounds; returns the number of still-active instances. */
+int jx_batch_run(jx_batch* b, int rounds, int* active_out);
+/* Per slot: done (1), running (0) or empty (-1), time and cycle. */
+int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap);
+/* Copy an instance's state to U_out (may be NULL) and return (time, last dt, cycle). */
+int jx_batch_result(jx_batch* b, int id, void* U_out, double* time, double* dt, int* cycle);
+int jx_batch_state(jx_batch* b, int id, void** state_out);
+int jx_batch_set_graph(jx_batch* b, int use_graph, int rounds_per_graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,7 +159,7 @@ struct LoopArgs {
   int nlim, smag_on, has_forcing, n_updates;
 };
 
-__global__ void init_ctrl_kernel(Ctrl* c) {
+__device__ inline void init_ctrl(Ctrl* c) {
   c->time = 0.0;
   c->dt = 0.0;
   c->prev_dt = __builtin_inf();
@@ -176,9 +176,11 @@ __global__ void init_ctrl_kernel(Ctrl* c) {
   c->sum_rho = 0.0;
 }
 
+static __global__ void init_ctrl_kernel(Ctrl* c) { init_ctrl(c); }
+
 // One thread: commit the previous step, test the loop condition (run.py:36-38), pick dt
 // (run.py:42-46) and the OU row due at the cycle-start time (forcing.py:125-134).
-__global__ void begin_step_kernel(Ctrl* c, LoopArgs p) {
+__device__ inline void begin_step(Ctrl* c, const LoopArgs& p) {
   if (c->done) return;
   if (c->pending) {
     c->time = c->time + c->dt;
@@ -209,8 +211,10 @@ __global__ void begin_step_kernel(Ctrl* c, LoopArgs p) {
   }
 }
 
+static __global__ void begin_step_kernel(Ctrl* c, LoopArgs p) { begin_step(c, p); }
+
 // Standalone step: dt (and the kick power) come from the caller.
-__global__ void set_step_kernel(Ctrl* c, double dt, double power) {
+static __global__ void set_step_kernel(Ctrl* c, double dt, double power) {
   c->dt = dt;
   c->done = 0;
   c->power = power;
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__(kAuxBlock) kick_first_moments_kernel(KickArgs<
 
 // Sum per-block partials in a fixed order.  mode 0: mean force mu = sum(rho r)/sum(rho)
 // (forcing.py:141).  mode 1: m0, m1 and the kick scale (forcing.py:144-148).
-__global__ void reduce_kick_kernel(Ctrl* c, const double* partial, int nblocks, int mode,
+static __global__ void reduce_kick_kernel(Ctrl* c, const double* partial, int nblocks, int mode,
                                    double ncells) {
   if (c->done) return;
   __shared__ double s[3][256];
@@ -495,7 +499,7 @@ __global__ void __launch_bounds__(kAuxBlock) box_sums_kernel(const T* u, size_t 
     for (int v = 0; v < nv; ++v) partial[(size_t)blockIdx.x * 8 + v] = s[v][0];
 }
 
-__global__ void box_sums_finish_kernel(const double* partial, int nblocks, int nv, double* out) {
+static __global__ void box_sums_finish_kernel(const double* partial, int nblocks, int nv, double* out) {
   if (threadIdx.x != 0) return;
   for (int v = 0; v < nv; ++v) {
     double a = 0.0;

@@ -1,0 +1,706 @@
+// Batched ensemble engine: many independent periodic grids of uneven sizes and mixed
+// operators advanced together by persistent wave-worker kernels.
+//
+// Work unit.  A tile is a 64-column strip (58 output columns plus the 3+3 halo) of one
+// instance over `rows` rows.  Each 64-lane wave is an independent worker: it takes the
+// next tile from a device counter, runs the instance's current RK stage on it with the
+// same column march as the single-grid kernel (x exchanges through the wave's own LDS
+// slice, so no block barrier couples workers), and moves on.  Reconstruction, Riemann
+// solver, viscosity and the number of variables are per-instance runtime choices, uniform
+// across a wave.
+//
+// Rounds.  One round advances every active instance by one RK stage of its own schedule:
+// control (begin-step for instances at stage 0) -> forcing render -> stage -> kick
+// (mean, moments, scale, apply; forced instances at their final stage) -> Smagorinsky
+// timestep -> advance.  All control lives in device memory, so a round is a fixed
+// sequence of launches that is captured once in a hipGraph and replayed.  Every
+// reduction is per instance and in a fixed order, so an instance's result does not
+// depend on which other instances share the batch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jx_aux.hpp"
+
+namespace jx {
+
+constexpr int kBatchNV = 4;            // variables per instance: hydro + one scalar
+constexpr int kBatchBlock = 256;       // four wave workers per block
+constexpr int kTileCols = 64 - 6;      // output columns per strip
+
+template <typename T> struct BatchInst {
+  T* A;                  // state between steps (floored after every step)
+  T* B; T* C; T* S;      // stage registers
+  T* raw;                // rendered raw force (forced instances)
+  const T* coef; const T* cx; const T* sx; const T* cy; const T* sy;
+  int nx, ny, nv, recon, rs, nstages;
+  int visc_on, smag_on, drag_on, forced, n_updates, n_modes;
+  int tile0, ntiles;
+  T dx, dy, cs, cs2, inv_cs, dfloor, eps, four_dx, four_dy, visc, smag_c, drag;
+  T inv_dx, inv_dy, inv_four_dx, inv_four_dy;
+  LoopArgs loop;
+};
+
+struct BatchCtrl {
+  Ctrl c;
+  int stage;    // RK stage this round executes
+  int active;   // takes part in this round
+  int live;     // slot holds an instance
+  int pad;
+};
+
+struct Tile { int inst, x0, y0, rows; };
+
+enum { CNT_RENDER = 0, CNT_STAGE, CNT_MOM, CNT_APPLY, CNT_TSTEP, CNT_TOTAL };
+
+template <typename T> struct BatchTabs {
+  BatchInst<T>* inst;
+  BatchCtrl* ctrl;
+  const Tile* tiles;
+  int* meta;              // [0] instance slots in use, [1] tiles, [2] active count
+  unsigned* counters;     // per-round work counters
+  double* partial;        // 3 doubles per tile
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Next tile index for this wave, as a wave-uniform scalar.  Every lane issues the atomic
+// (lane 0 adds 1, the others 0) and the wave takes lane 0's old value, so the loop header
+// holds no lane-divergent branch and every exit of the worker loops is a scalar branch.
+__device__ __forceinline__ int wave_next(unsigned* counter) {
+  const unsigned inc = ((threadIdx.x & 63) == 0) ? 1u : 0u;
+  const unsigned old = atomicAdd(counter, inc);
+  return __builtin_amdgcn_readfirstlane((int)old);
+}
+
+// RK rows (hydro.py:20-36) by number of stages and stage index.
+__device__ __forceinline__ void rk_row(int ns, int k, double& g0, double& g1, double& beta,
+                                       double& delta, int& has_delta) {
+  has_delta = 0;
+  delta = 0.0;
+  if (ns == 2) {
+    g0 = k == 0 ? 0.0 : 0.5;
+    g1 = k == 0 ? 1.0 : 0.5;
+    beta = k == 0 ? 1.0 : 0.5;
+  } else if (ns == 3) {
+    g0 = k == 0 ? 0.0 : k == 1 ? 0.25 : 2.0 / 3.0;
+    g1 = k == 0 ? 1.0 : k == 1 ? 0.75 : 1.0 / 3.0;
+    beta = k == 0 ? 1.0 : k == 1 ? 0.25 : 2.0 / 3.0;
+  } else {
+    const double G0[4] = {0.0, 0.121098479554482, -3.843833699660025, 0.546370891121863};
+    const double G1[4] = {1.0, 0.721781678111411, 2.121209265338722, 0.198653035682705};
+    const double BE[4] = {1.193743905974738, 0.099279895495783, 1.131678018054042,
+                          0.310665766509336};
+    const double DE[4] = {0.0, 0.217683334308543, 1.065841341361089, 0.0};
+    g0 = G0[k];
+    g1 = G1[k];
+    beta = BE[k];
+    delta = DE[k];
+    has_delta = k > 0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void edges_rt(int recon, T qm2, T qm1, T q, T qp1, T qp2, T eps,
+                                         T& lo, T& hi) {
+  switch (recon) {
+    case RECON_DC: cell_edges<RECON_DC>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
+    case RECON_PLM: cell_edges<RECON_PLM>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
+    case RECON_PPM4: cell_edges<RECON_PPM4>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
+    case RECON_PPMX: cell_edges<RECON_PPMX>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
+    default: cell_edges<RECON_WENOZ>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void riemann_rt(int rs, T rl, T rr, T ul, T ur, T tl, T tr,
+                                           const Gas<T>& g, T& f0, T& f1, T& f2) {
+  switch (rs) {
+    case RS_LLF: riemann<RS_LLF, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
+    case RS_HLLE: riemann<RS_HLLE, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
+    default: riemann<RS_ROE, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T over_rt(T x, T d, T inv) {
+  if constexpr (Fast<T>::value) return x * inv;
+  else return x / d;
+}
+
+// One RK stage of one instance over one tile, executed by one wave.  Same arithmetic as
+// stage_kernel; `lds` is this wave's slice (21 columns of 64 values).
+template <typename T>
+__device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, const Tile& tl,
+                                 int k_tile, T* lds, double* partial) {
+  constexpr int NV = kBatchNV;
+  constexpr int R = 5;
+  const int t = threadIdx.x & 63;
+  T* sW = lds;             // [NV][64]
+  T* sH = sW + NV * 64;    // [NV][64]
+  T* sF = sH + NV * 64;    // [NV][64]
+  T* sV = sF + NV * 64;    // [3][3][64]
+  const int nx = I.nx, ny = I.ny, nv = I.nv;
+  const bool nv4 = nv > 3;
+  const int x0 = tl.x0;
+  int xc = (x0 + t - 3) % nx;
+  if (xc < 0) xc += nx;
+  const bool col_out = (t >= 3) && (t < 61) && (x0 + t - 3 < nx);
+  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, 63), tp2 = min(t + 2, 63);
+  const int y0 = tl.y0;
+  const int yend = min(y0 + tl.rows, ny);
+  const int nit = (yend - y0) + 6;
+
+  // Register plan of the stage (all saved-register reads are pointwise):
+  //   rk2: A->B, B|A->A   rk3: A->B, B|A->C, C|A->A
+  //   rk4: A->B, B|A->C (S=A+d2 B), C|S->B (S+=d3 C), B|S->A (S+=0 B)
+  const int ns = I.nstages, k = C.stage;
+  double g0, g1, beta, delta;
+  int has_delta;
+  rk_row(ns, k, g0, g1, beta, delta, has_delta);
+  const bool blended = (g0 != 0.0 && g0 + g1 == 1.0);
+  const bool last = (k == ns - 1);
+  const T* u_in = I.A;
+  T* u_out = I.B;
+  const T* u_sv = nullptr;
+  T* s_out = nullptr;
+  bool floor_saved = false;
+  if (k > 0) {
+    if (ns == 2) { u_in = I.B; u_out = I.A; }
+    else if (ns == 3) { u_in = (k == 1) ? I.B : I.C; u_out = (k == 1) ? I.C : I.A; }
+    else {
+      u_in = (k == 1) ? I.B : (k == 2) ? I.C : I.B;
+      u_out = (k == 1) ? I.C : (k == 2) ? I.B : I.A;
+      s_out = I.S;
+    }
+    const bool from_a = (ns != 4) || (k == 1);
+    u_sv = from_a ? I.A : I.S;
+    floor_saved = from_a;
+  }
+  const bool kick = last && I.forced;
+  const bool want_key = last && !I.forced && !I.smag_on;
+
+  const double dt = C.c.dt;
+  const T sdt = T(beta * dt);
+  const Gas<T> gas{I.cs, I.cs2, I.inv_cs};
+  const Visc<T> vp{I.visc, I.smag_c, I.smag_on != 0};
+  const T dfloor = I.dfloor;
+  const T drag_coef = sdt * I.drag;
+  const bool visc_on = I.visc_on != 0;
+  const bool floors = I.recon == RECON_PPMX || I.recon == RECON_WENOZ;
+
+  T w[R][NV], u[R][NV], hy[NV], gp[NV], gn[NV], pf[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    hy[v] = gp[v] = gn[v] = pf[v] = T(0);
+#pragma unroll
+    for (int q = 0; q < R; ++q) w[q][v] = u[q][v] = T(0);
+  }
+  {
+    int yn = (y0 - 3) % ny;
+    if (yn < 0) yn += ny;
+    const T* src = u_in + ((size_t)yn * nx + xc) * nv;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      if (v < 3 || nv4) pf[v] = src[v];
+  }
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  unsigned long long kmin = ~0ull;
+
+  auto body = [&](auto pc, int i) {
+    constexpr int P = decltype(pc)::value;
+    constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
+    constexpr int S4 = (P + R - 4) % R;
+    const int r = y0 - 3 + i;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) u[S0][v] = pf[v];
+    if (i + 1 < nit) {
+      int yn = (r + 1) % ny;
+      if (yn < 0) yn += ny;
+      const T* src = u_in + ((size_t)yn * nx + xc) * nv;
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        if (v < 3 || nv4) pf[v] = src[v];
+    }
+    T svr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) svr[v] = T(0);
+    if (i >= 6 && u_sv) {
+      const T* sv = u_sv + ((size_t)(r - 3) * nx + xc) * nv;
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        if (v < 3 || nv4) svr[v] = sv[v];
+    }
+    {
+      const T rho = (u[S0][0] < dfloor) ? dfloor : u[S0][0];
+      u[S0][0] = rho;
+      const T inv = dvd(T(1), rho);
+      w[S0][0] = rho;
+#pragma unroll
+      for (int v = 1; v < NV; ++v) w[S0][v] = inv * u[S0][v];
+    }
+    if (visc_on && i >= 2) {
+      T* slot = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * 64;
+      slot[t] = w[S2][0];
+      slot[64 + t] = w[S2][1];
+      slot[128 + t] = w[S2][2];
+    }
+    if (i >= 6) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sW[v * 64 + t] = w[S3][v];
+    }
+    wave_sync();
+
+    if (i >= 4) {
+      T lo[NV], hi[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        edges_rt<T>(I.recon, w[S4][v], w[S3][v], w[S2][v], w[S1][v], w[S0][v], I.eps, lo[v],
+                    hi[v]);
+      if (floors) {
+        lo[0] = fmx(lo[0], dfloor);
+        hi[0] = fmx(hi[0], dfloor);
+      }
+      if (i >= 5) {
+        T f0, f1, f2;
+        riemann_rt<T>(I.rs, hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
+        gn[0] = f0;
+        gn[1] = f2;
+        gn[2] = f1;
+        gn[3] = f0 * ((f0 >= T(0)) ? hy[3] : lo[3]);
+        if (visc_on) {
+          const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * 64;
+          const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * 64;
+          const T uy = over_rt(w[S2][1] - w[S3][1], I.dy, I.inv_dy);
+          const T vy = over_rt(w[S2][2] - w[S3][2], I.dy, I.inv_dy);
+          const T ux = over_rt((rs[64 + tp1] + rm[64 + tp1]) - (rs[64 + tm1] + rm[64 + tm1]),
+                               I.four_dx, I.inv_four_dx);
+          const T vx = over_rt((rs[128 + tp1] + rm[128 + tp1]) - (rs[128 + tm1] + rm[128 + tm1]),
+                               I.four_dx, I.inv_four_dx);
+          T fmx_, fmy_;
+          visc_face<0>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
+          gn[1] = gn[1] + fmx_;
+          gn[2] = gn[2] + fmy_;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) hy[v] = hi[v];
+    }
+
+    if (i >= 6) {
+      T xl[NV], xh[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const T* row = sW + v * 64;
+        edges_rt<T>(I.recon, row[tm2], row[tm1], row[t], row[tp1], row[tp2], I.eps, xl[v], xh[v]);
+      }
+      if (floors) {
+        xl[0] = fmx(xl[0], dfloor);
+        xh[0] = fmx(xh[0], dfloor);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sH[v * 64 + t] = xh[v];
+      wave_sync();
+      T fx[NV];
+      {
+        T l[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) l[v] = sH[v * 64 + tm1];
+        T f0, f1, f2;
+        riemann_rt<T>(I.rs, l[0], xl[0], l[1], xl[1], l[2], xl[2], gas, f0, f1, f2);
+        fx[0] = f0;
+        fx[1] = f1;
+        fx[2] = f2;
+        fx[3] = f0 * ((f0 >= T(0)) ? l[3] : xl[3]);
+        if (visc_on) {
+          const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * 64;
+          const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * 64;
+          const T* rq = sV + ((r - 4 + 3 * ny + 3) % 3) * 3 * 64;
+          const T ux = over_rt(rj[64 + t] - rj[64 + tm1], I.dx, I.inv_dx);
+          const T vx = over_rt(rj[128 + t] - rj[128 + tm1], I.dx, I.inv_dx);
+          const T uy = over_rt((rp[64 + t] + rp[64 + tm1]) - (rq[64 + t] + rq[64 + tm1]),
+                               I.four_dy, I.inv_four_dy);
+          const T vy = over_rt((rp[128 + t] + rp[128 + tm1]) - (rq[128 + t] + rq[128 + tm1]),
+                               I.four_dy, I.inv_four_dy);
+          T fmx_, fmy_;
+          visc_face<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
+          fx[1] = fx[1] + fmx_;
+          fx[2] = fx[2] + fmy_;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sF[v * 64 + t] = fx[v];
+      wave_sync();
+      if (col_out) {
+        const int j = r - 3;
+        const size_t cell = (size_t)j * nx + (x0 + t - 3);
+        T un[NV], us[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) us[v] = (k == 0) ? u[S3][v] : svr[v];
+        if (k > 0 && floor_saved) us[0] = (us[0] < dfloor) ? dfloor : us[0];
+        if (has_delta) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            us[v] = us[v] + T(delta) * u[S3][v];
+            if (v < 3 || nv4) s_out[cell * nv + v] = us[v];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const T div = over_rt(sF[v * 64 + tp1] - fx[v], I.dx, I.inv_dx) +
+                        over_rt(gn[v] - gp[v], I.dy, I.inv_dy);
+          if (blended)
+            un[v] = us[v] + (T(g0) * (u[S3][v] - us[v]) - sdt * div);
+          else
+            un[v] = (T(g0) * u[S3][v] + T(g1) * us[v]) - sdt * div;
+        }
+        if (I.drag_on) {
+          const T c = drag_coef * w[S3][0];
+          un[1] = un[1] + (-(c * w[S3][1]));
+          un[2] = un[2] + (-(c * w[S3][2]));
+        }
+        if (kick) {
+          const T* rf = I.raw + cell * 2;
+          acc0 += (double)un[0];
+          acc1 += (double)(un[0] * rf[0]);
+          acc2 += (double)(un[0] * rf[1]);
+        } else {
+          un[0] = (un[0] < dfloor) ? dfloor : un[0];
+          if (want_key) {
+            const T inv = dvd(T(1), un[0]);
+            const T lx = dvd<T>(I.dx, T(fabs(inv * un[1])) + I.cs);
+            const T ly = dvd<T>(I.dy, T(fabs(inv * un[2])) + I.cs);
+            const unsigned long long kx = key_of((double)lx), ky = key_of((double)ly);
+            kmin = kx < kmin ? kx : kmin;
+            kmin = ky < kmin ? ky : kmin;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          if (v < 3 || nv4) u_out[cell * nv + v] = un[v];
+      }
+    }
+    if (i >= 5) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) gp[v] = gn[v];
+    }
+  };
+
+  for (int i0 = 0; i0 < nit; i0 += R) {
+    body(IC<0>{}, i0);
+    if (i0 + 1 < nit) body(IC<1>{}, i0 + 1);
+    if (i0 + 2 < nit) body(IC<2>{}, i0 + 2);
+    if (i0 + 3 < nit) body(IC<3>{}, i0 + 3);
+    if (i0 + 4 < nit) body(IC<4>{}, i0 + 4);
+  }
+  // Wave reductions (fixed butterfly): kick moments per tile, or the CFL key.
+  for (int off = 32; off > 0; off >>= 1) {
+    acc0 += __shfl_xor(acc0, off);
+    acc1 += __shfl_xor(acc1, off);
+    acc2 += __shfl_xor(acc2, off);
+    const unsigned long long o = __shfl_xor(kmin, off);
+    kmin = o < kmin ? o : kmin;
+  }
+  if (t == 0) {
+    if (kick) {
+      partial[3 * (size_t)k_tile + 0] = acc0;
+      partial[3 * (size_t)k_tile + 1] = acc1;
+      partial[3 * (size_t)k_tile + 2] = acc2;
+    } else if (want_key) {
+      atomicMin(const_cast<unsigned long long*>(&C.c.dtkey), kmin);
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBatchBlock) batch_stage_kernel(BatchTabs<T> b) {
+  extern __shared__ double bsm[];
+  T* lds = reinterpret_cast<T*>(bsm) + (threadIdx.x >> 6) * (3 * kBatchNV + 9) * 64;
+  const int ntiles = b.meta[1];
+  for (;;) {
+    const int kt = wave_next(&b.counters[CNT_STAGE]);
+    if (kt >= ntiles) break;
+    const Tile tl = b.tiles[kt];
+    const BatchCtrl& C = b.ctrl[tl.inst];
+    if (C.active) batch_stage_tile<T>(b.inst[tl.inst], C, tl, kt, lds, b.partial);
+  }
+}
+
+// Control: commit/open steps for instances at stage 0, decide who runs, reset counters.
+template <typename T>
+__global__ void batch_control_kernel(BatchTabs<T> b) {
+  if (threadIdx.x < CNT_TOTAL) b.counters[threadIdx.x] = 0u;
+  __shared__ int active;
+  if (threadIdx.x == 0) active = 0;
+  __syncthreads();
+  const int n = b.meta[0];
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    BatchCtrl& C = b.ctrl[s];
+    if (C.live) {
+      if (C.stage == 0) begin_step(&C.c, b.inst[s].loop);
+      C.active = !C.c.done;
+      if (C.active) atomicAdd(&active, 1);
+    } else {
+      C.active = 0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) b.meta[2] = active;
+}
+
+template <typename T>
+__global__ void batch_advance_kernel(BatchTabs<T> b) {
+  const int n = b.meta[0];
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    BatchCtrl& C = b.ctrl[s];
+    if (C.live && C.active) C.stage = (C.stage + 1) % b.inst[s].nstages;
+  }
+}
+
+template <typename T>
+__device__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C, const Tile& tl) {
+  const int t = threadIdx.x & 63;
+  const int row = C.c.update;
+  const bool fill = (row < 0) || (row >= I.n_updates);
+  const T* amp = I.coef + (size_t)(fill ? 0 : row) * I.n_modes * 4;
+  const T nan = T(__builtin_nan(""));
+  const int i = tl.x0 + t;
+  const bool on = t < kTileCols && i < I.nx;   // per lane; the tile loop stays uniform
+  const int yend = on ? min(tl.y0 + tl.rows, I.ny) : tl.y0;
+  for (int j = tl.y0; j < yend; ++j) {
+    T f0 = T(0), f1 = T(0);
+    for (int m = 0; m < I.n_modes; ++m) {
+      const T cxv = I.cx[(size_t)m * I.nx + i], sxv = I.sx[(size_t)m * I.nx + i];
+      const T cyv = I.cy[(size_t)m * I.ny + j], syv = I.sy[(size_t)m * I.ny + j];
+      const T re = cxv * cyv - sxv * syv;
+      const T im = cxv * syv + sxv * cyv;
+      const T* am = amp + (size_t)m * 4;
+      const T a00 = fill ? nan : am[0], a01 = fill ? nan : am[1];
+      const T a10 = fill ? nan : am[2], a11 = fill ? nan : am[3];
+      f0 = f0 + (re * a00 - im * a01);
+      f1 = f1 + (re * a10 - im * a11);
+    }
+    I.raw[((size_t)j * I.nx + i) * 2 + 0] = f0;
+    I.raw[((size_t)j * I.nx + i) * 2 + 1] = f1;
+  }
+}
+
+// Raw-force render on the tiles of instances that opened a step with a new OU row.
+template <typename T>
+__global__ void __launch_bounds__(kBatchBlock) batch_render_kernel(BatchTabs<T> b) {
+  const int ntiles = b.meta[1];
+  for (;;) {
+    const int kt = wave_next(&b.counters[CNT_RENDER]);
+    if (kt >= ntiles) break;
+    const Tile tl = b.tiles[kt];
+    const BatchCtrl& C = b.ctrl[tl.inst];
+    const BatchInst<T>& I = b.inst[tl.inst];
+    if (C.active && C.stage == 0 && I.forced && C.c.render) batch_render_tile<T>(I, C, tl);
+  }
+}
+
+__device__ __forceinline__ bool kicking(const BatchCtrl& C, int forced, int nstages) {
+  return C.active && forced && C.stage == nstages - 1;
+}
+
+// Per-instance fixed-order sums of tile partials.  mode 0: mean force; mode 1: scale.
+template <typename T>
+__global__ void batch_kick_reduce_kernel(BatchTabs<T> b, int mode) {
+  const int n = b.meta[0];
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    BatchCtrl& C = b.ctrl[s];
+    const BatchInst<T>& I = b.inst[s];
+    if (!kicking(C, I.forced, I.nstages)) continue;   // per-thread loop over instances
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int k = I.tile0; k < I.tile0 + I.ntiles; ++k) {
+      a0 += b.partial[3 * (size_t)k + 0];
+      a1 += b.partial[3 * (size_t)k + 1];
+      a2 += b.partial[3 * (size_t)k + 2];
+    }
+    if (mode == 0) {
+      C.c.sum_rho = a0;
+      C.c.mu[0] = a1 / a0;
+      C.c.mu[1] = a2 / a0;
+    } else {
+      const double ncell = (double)I.nx * I.ny;
+      const double m0 = a0 / ncell, m1 = a1 / ncell;
+      const double dedt = C.c.power;
+      const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
+      double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
+      if (!(m0 > 1e-30)) scale = __builtin_nan("");
+      C.c.scale = scale;
+    }
+  }
+}
+
+// Second moments (mode 0) or the kick itself (mode 1) on one tile of a kicking instance.
+template <typename T>
+__device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl, int kt,
+                                int mode, double* partial) {
+  const int t = threadIdx.x & 63;
+  const T mu0 = T(C.c.mu[0]), mu1 = T(C.c.mu[1]);
+  const T dt = T(C.c.dt);
+  const bool off = C.c.power == 0.0;
+  const T scale = T(C.c.scale);
+  const int i = tl.x0 + t;
+  const bool on = t < kTileCols && i < I.nx;
+  const int yend = on ? min(tl.y0 + tl.rows, I.ny) : tl.y0;
+  double s0 = 0, s1 = 0;
+  unsigned long long kmin = ~0ull;
+  for (int j = tl.y0; j < yend; ++j) {
+    const size_t c = (size_t)j * I.nx + i;
+    T* q = I.A + c * I.nv;
+    const T f0 = I.raw[c * 2] - mu0, f1 = I.raw[c * 2 + 1] - mu1;
+    if (mode == 0) {
+      const T fsq = f0 * f0 + f1 * f1;
+      s0 += (double)(((T(0.5) * q[0]) * fsq) * dt);
+      s1 += (double)(q[1] * f0 + q[2] * f1);
+    } else {
+      const T acc0 = off ? T(0) : scale * f0;
+      const T acc1 = off ? T(0) : scale * f1;
+      const T rho = q[0];
+      const T m0 = q[1] + rho * (acc0 * dt);
+      const T m1 = q[2] + rho * (acc1 * dt);
+      const T rf = (rho < I.dfloor) ? I.dfloor : rho;
+      q[0] = rf;
+      q[1] = m0;
+      q[2] = m1;
+      if (!I.smag_on) {
+        const T inv = dvd(T(1), rf);
+        const unsigned long long kx = key_of((double)dvd<T>(I.dx, T(fabs(inv * m0)) + I.cs));
+        const unsigned long long ky = key_of((double)dvd<T>(I.dy, T(fabs(inv * m1)) + I.cs));
+        kmin = kx < kmin ? kx : kmin;
+        kmin = ky < kmin ? ky : kmin;
+      }
+    }
+  }
+  for (int off2 = 32; off2 > 0; off2 >>= 1) {
+    s0 += __shfl_xor(s0, off2);
+    s1 += __shfl_xor(s1, off2);
+    const unsigned long long o = __shfl_xor(kmin, off2);
+    kmin = o < kmin ? o : kmin;
+  }
+  if (t == 0) {
+    if (mode == 0) {
+      partial[3 * (size_t)kt + 0] = s0;
+      partial[3 * (size_t)kt + 1] = s1;
+      partial[3 * (size_t)kt + 2] = 0.0;
+    } else if (!I.smag_on) {
+      atomicMin(&C.c.dtkey, kmin);
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBatchBlock) batch_kick_tiles_kernel(BatchTabs<T> b, int mode) {
+  const int ntiles = b.meta[1];
+  for (;;) {
+    const int kt = wave_next(&b.counters[mode == 0 ? CNT_MOM : CNT_APPLY]);
+    if (kt >= ntiles) break;
+    const Tile tl = b.tiles[kt];
+    BatchCtrl& C = b.ctrl[tl.inst];
+    const BatchInst<T>& I = b.inst[tl.inst];
+    if (kicking(C, I.forced, I.nstages)) batch_kick_tile<T>(I, C, tl, kt, mode, b.partial);
+  }
+}
+
+// CFL and Smagorinsky limits over one tile of an instance's current state A.
+template <typename T>
+__device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl) {
+  const int t = threadIdx.x & 63;
+  const int i = tl.x0 + t;
+  const bool on = t < kTileCols && i < I.nx;
+  const int nx = I.nx, ny = I.ny, nv = I.nv;
+  unsigned long long kmin = ~0ull, kmax = 0ull;
+  const int yend = on ? min(tl.y0 + tl.rows, ny) : tl.y0;
+  for (int j = tl.y0; j < yend; ++j) {
+    T r, vx, vy;
+    load_prim(I.A + ((size_t)j * nx + i) * nv, nv, false, I.dfloor, r, vx, vy);
+    const unsigned long long kx = key_of((double)dvd(I.dx, fabs(vx) + I.cs));
+    const unsigned long long ky = key_of((double)dvd(I.dy, fabs(vy) + I.cs));
+    kmin = kx < kmin ? kx : kmin;
+    kmin = ky < kmin ? ky : kmin;
+    if (I.smag_on) {
+      const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
+      const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+      auto at = [&](int jj, int ii, T& u_, T& v_) {
+        T rr;
+        load_prim(I.A + ((size_t)jj * nx + ii) * nv, nv, false, I.dfloor, rr, u_, v_);
+      };
+      T u_w, v_w, u_n, v_n, u_nw, v_nw, u_s, v_s, u_sw, v_sw, u_e, v_e, u_se, v_se;
+      at(j, im, u_w, v_w);
+      at(jp, i, u_n, v_n);
+      at(jp, im, u_nw, v_nw);
+      at(jm, i, u_s, v_s);
+      at(jm, im, u_sw, v_sw);
+      at(j, ip, u_e, v_e);
+      at(jm, ip, u_se, v_se);
+      {
+        const T ux = dvd(vx - u_w, I.dx);
+        const T vxn = dvd(vy - v_w, I.dx);
+        const T uy = dvd((u_n + u_nw) - (u_s + u_sw), I.four_dy);
+        const T vyt = dvd((v_n + v_nw) - (v_s + v_sw), I.four_dy);
+        const unsigned long long k = max_key_of((double)smag_nu(ux, vyt, vxn + uy, I.smag_c));
+        kmax = k > kmax ? k : kmax;
+      }
+      {
+        const T uy = dvd(vx - u_s, I.dy);
+        const T vyn = dvd(vy - v_s, I.dy);
+        const T uxt = dvd((u_e + u_se) - (u_w + u_sw), I.four_dx);
+        const T vxt = dvd((v_e + v_se) - (v_w + v_sw), I.four_dx);
+        const unsigned long long k = max_key_of((double)smag_nu(uxt, vyn, vxt + uy, I.smag_c));
+        kmax = k > kmax ? k : kmax;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(kmin, off);
+    kmin = o < kmin ? o : kmin;
+    const unsigned long long p = __shfl_xor(kmax, off);
+    kmax = p > kmax ? p : kmax;
+  }
+  if (t == 0) {
+    atomicMin(&C.c.dtkey, kmin);
+    if (I.smag_on) atomicMax(&C.c.nukey, kmax);
+  }
+}
+
+// `only` >= 0 selects one instance (its initial timestep); otherwise Smagorinsky instances
+// that have just finished a step.
+template <typename T>
+__global__ void __launch_bounds__(kBatchBlock) batch_timestep_kernel(BatchTabs<T> b, int only) {
+  const int ntiles = b.meta[1];
+  for (;;) {
+    const int kt = wave_next(&b.counters[CNT_TSTEP]);
+    if (kt >= ntiles) break;
+    const Tile tl = b.tiles[kt];
+    BatchCtrl& C = b.ctrl[tl.inst];
+    const BatchInst<T>& I = b.inst[tl.inst];
+    const bool go = (only >= 0) ? (tl.inst == only)
+                                : (C.active && I.smag_on && C.stage == I.nstages - 1);
+    if (go) batch_timestep_tile<T>(I, C, tl);
+  }
+}
+
+template <typename T>
+__global__ void batch_init_kernel(BatchTabs<T> b, int slot) {
+  BatchCtrl& C = b.ctrl[slot];
+  init_ctrl(&C.c);
+  C.stage = 0;
+  C.active = 0;
+  C.live = 1;
+  b.counters[CNT_TSTEP] = 0u;
+}
+
+template <typename T>
+__global__ void batch_retire_kernel(BatchTabs<T> b, int slot) {
+  b.ctrl[slot].live = 0;
+  b.ctrl[slot].active = 0;
+}
+
+static __global__ void reset_counter_kernel(unsigned* c) { *c = 0u; }
+
+}  // namespace jx

@@ -1,0 +1,440 @@
+// Host side of the batched engine: instance slots in one HBM pool, the tile list, the
+// per-round launch sequence and its hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/jathenax_amd.h"
+#include "jx_batch.hpp"
+
+namespace jx {
+
+struct BatchBase {
+  virtual ~BatchBase() {}
+  std::string err;
+  virtual int add(const jx_run_params* p, const void* U0, const jx_forcing* f, int rows,
+                  int* id) = 0;
+  virtual int remove(int id) = 0;
+  virtual int run(int rounds, int* active) = 0;
+  virtual int poll(int* done, double* time, int* cycle, int cap) = 0;
+  virtual int result(int id, void* U_out, double* time, double* dt, int* cycle) = 0;
+  virtual int state_ptr(int id, void** p) = 0;
+  virtual int set_graph(int on, int rounds) = 0;
+  virtual long long pool_free() const = 0;
+  virtual long long footprint(const jx_run_params* p, int forced) const = 0;
+};
+
+inline bool batch_pow2(double x) {
+  if (!(x > 0.0) || !std::isfinite(x)) return false;
+  int e = 0;
+  return std::frexp(x, &e) == 0.5;
+}
+
+template <typename T> class Batch : public BatchBase {
+ public:
+  Batch(int device, int max_inst, long long max_tiles, long long pool_bytes)
+      : device_(device), max_inst_(max_inst), max_tiles_(max_tiles), pool_bytes_(pool_bytes) {}
+
+  int init() {
+    if (hipSetDevice(device_) != hipSuccess) return fail("hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess)
+      return fail("stream creation failed");
+    hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<T>,
+                                                     kBatchBlock, smem()) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    grid_ = std::max(1, cus_) * per_cu;
+    if (hipMalloc(&d_inst_, sizeof(BatchInst<T>) * max_inst_) != hipSuccess ||
+        hipMalloc(&d_ctrl_, sizeof(BatchCtrl) * max_inst_) != hipSuccess ||
+        hipMalloc(&d_tiles_, sizeof(Tile) * max_tiles_) != hipSuccess ||
+        hipMalloc(&d_meta_, sizeof(int) * 8) != hipSuccess ||
+        hipMalloc(&d_cnt_, sizeof(unsigned) * 8) != hipSuccess ||
+        hipMalloc(&d_partial_, sizeof(double) * 3 * max_tiles_) != hipSuccess ||
+        hipMalloc(&d_pool_, pool_bytes_) != hipSuccess ||
+        hipHostMalloc(&h_ctrl_, sizeof(BatchCtrl) * max_inst_, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc(&h_meta_, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess)
+      return fail("batch allocation failed (pool too large for the device?)");
+    hipMemsetAsync(d_ctrl_, 0, sizeof(BatchCtrl) * max_inst_, stream_);
+    hipMemsetAsync(d_meta_, 0, sizeof(int) * 8, stream_);
+    hipMemsetAsync(d_cnt_, 0, sizeof(unsigned) * 8, stream_);
+    free_[0] = (size_t)pool_bytes_;
+    h_inst_.resize(max_inst_);
+    slots_.assign(max_inst_, Slot{});
+    if (hipStreamSynchronize(stream_) != hipSuccess) return fail("init sync failed");
+    return JX_OK;
+  }
+
+  ~Batch() override {
+    if (stream_) hipStreamSynchronize(stream_);
+    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+    if (graph_) hipGraphDestroy(graph_);
+    for (void* p : {(void*)d_inst_, (void*)d_ctrl_, (void*)d_tiles_, (void*)d_meta_,
+                    (void*)d_cnt_, (void*)d_partial_, (void*)d_pool_})
+      if (p) hipFree(p);
+    if (h_ctrl_) hipHostFree(h_ctrl_);
+    if (h_meta_) hipHostFree(h_meta_);
+    if (stream_) hipStreamDestroy(stream_);
+  }
+
+  long long footprint(const jx_run_params* p, int forced) const override {
+    const size_t n = (size_t)p->nx * p->ny;
+    size_t b = 4 * align(n * p->nvar * sizeof(T));
+    if (forced) b += align(n * 2 * sizeof(T));
+    return (long long)b;
+  }
+
+  long long pool_free() const override {
+    long long f = 0;
+    for (auto& kv : free_) f += (long long)kv.second;
+    return f;
+  }
+
+  int add(const jx_run_params* p, const void* U0, const jx_forcing* f, int rows,
+          int* id) override {
+    if (!p || !U0 || !id) return fail("add: null argument");
+    if (p->dtype != (Fast<T>::value ? 1 : 0)) return fail("add: dtype does not match batch");
+    if (p->nx < 1 || p->ny < 1) return fail("add: grid must be non-empty");
+    if (p->nvar < 3 || p->nvar > kBatchNV) return fail("add: batched instances have nvar 3 or 4");
+    if (p->reconstruct < 0 || p->reconstruct > 4 || p->rsolver < 0 || p->rsolver > 2 ||
+        p->integrator < 2 || p->integrator > 4)
+      return fail("add: unknown operator id", JX_EKEY);
+    for (double v : {p->viscosity, p->smagorinsky_cs, p->drag_rate, p->dedt})
+      if (!std::isfinite(v) || v < 0.0) return fail("add: coefficients must be finite and >= 0");
+    if (f && (!f->coefficients || !f->cos_x || !f->sin_x || !f->cos_y || !f->sin_y ||
+              f->n_modes < 1 || f->n_updates < 1 || !(p->dt_update > 0.0)))
+      return fail("add: incomplete forcing");
+    int s = -1;
+    for (int k = 0; k < max_inst_; ++k)
+      if (!slots_[k].live) { s = k; break; }
+    if (s < 0) return fail("add: no free instance slot");
+    const size_t n = (size_t)p->nx * p->ny;
+    const size_t sb = align(n * p->nvar * sizeof(T)), rb = f ? align(n * 2 * sizeof(T)) : 0;
+    const size_t need = 4 * sb + rb;
+    long long off = alloc(need);
+    if (off < 0) return fail("add: pool exhausted");
+    const int r = rows > 0 ? rows : 32;
+    const long long strips = (p->nx + kTileCols - 1) / kTileCols;
+    const long long segs = (p->ny + r - 1) / r;
+    if (ntiles_total() + strips * segs > max_tiles_) {
+      release((size_t)off, need);
+      return fail("add: tile list full");
+    }
+    Slot& sl = slots_[s];
+    sl = Slot{};
+    sl.live = true;
+    sl.off = (size_t)off;
+    sl.bytes = need;
+    sl.rows = r;
+    sl.p = *p;
+    char* base = static_cast<char*>(d_pool_) + off;
+    BatchInst<T> I{};
+    I.A = reinterpret_cast<T*>(base);
+    I.B = reinterpret_cast<T*>(base + sb);
+    I.C = reinterpret_cast<T*>(base + 2 * sb);
+    I.S = reinterpret_cast<T*>(base + 3 * sb);
+    I.raw = f ? reinterpret_cast<T*>(base + 4 * sb) : nullptr;
+    if (f) {
+      I.coef = static_cast<const T*>(f->coefficients);
+      I.cx = static_cast<const T*>(f->cos_x);
+      I.sx = static_cast<const T*>(f->sin_x);
+      I.cy = static_cast<const T*>(f->cos_y);
+      I.sy = static_cast<const T*>(f->sin_y);
+      I.n_updates = f->n_updates;
+      I.n_modes = f->n_modes;
+    }
+    I.nx = p->nx;
+    I.ny = p->ny;
+    I.nv = p->nvar;
+    I.recon = p->reconstruct;
+    I.rs = p->rsolver;
+    I.nstages = p->integrator;
+    I.visc_on = p->viscosity > 0.0 || p->smagorinsky_cs > 0.0;
+    I.smag_on = p->smagorinsky_cs > 0.0;
+    I.drag_on = p->drag_rate > 0.0;
+    I.forced = f != nullptr;
+    I.dx = T(p->dx);
+    I.dy = T(p->dy);
+    I.cs = T(p->cs);
+    I.cs2 = T(p->cs * p->cs);
+    I.inv_cs = T(1.0 / p->cs);
+    I.dfloor = T(p->dfloor);
+    I.eps = T(std::max(1e-42, (double)std::numeric_limits<T>::min()));
+    I.four_dx = T(4.0 * p->dx);
+    I.four_dy = T(4.0 * p->dy);
+    I.visc = T(p->viscosity);
+    I.smag_c = T(p->smagorinsky_cs * p->smagorinsky_cs * p->dx * p->dy);
+    I.drag = T(p->drag_rate);
+    I.inv_dx = T(1.0 / p->dx);
+    I.inv_dy = T(1.0 / p->dy);
+    I.inv_four_dx = T(1.0 / (4.0 * p->dx));
+    I.inv_four_dy = T(1.0 / (4.0 * p->dy));
+    LoopArgs& L = I.loop;
+    L.tlim = p->tlim;
+    L.cfl = p->cfl;
+    L.viscosity = p->viscosity;
+    L.drag_rate = p->drag_rate;
+    L.h = std::min(p->dx, p->dy);
+    L.dt_update = p->dt_update;
+    L.tdriv_start = p->tdriv_start;
+    L.dedt = f ? p->dedt : 0.0;
+    L.nlim = p->nlim;
+    L.smag_on = I.smag_on;
+    L.has_forcing = I.forced;
+    L.n_updates = I.n_updates;
+    h_inst_[s] = I;
+    if (hipMemcpyAsync(I.A, U0, n * p->nvar * sizeof(T), hipMemcpyDeviceToDevice, stream_) !=
+        hipSuccess)
+      return fail("add: state copy failed");
+    if (I.raw) hipMemsetAsync(I.raw, 0, n * 2 * sizeof(T), stream_);
+    int rc = rebuild_tiles();
+    if (rc) return rc;
+    BatchTabs<T> tb = tabs();
+    hipLaunchKernelGGL(batch_init_kernel<T>, dim3(1), dim3(1), 0, stream_, tb, s);
+    hipLaunchKernelGGL(batch_timestep_kernel<T>, dim3(grid_), dim3(kBatchBlock), 0, stream_, tb,
+                       s);
+    if (hipGetLastError() != hipSuccess) return fail("add: init launch failed");
+    *id = s;
+    return JX_OK;
+  }
+
+  int remove(int id) override {
+    if (id < 0 || id >= max_inst_ || !slots_[id].live) return fail("remove: no such instance");
+    hipLaunchKernelGGL(batch_retire_kernel<T>, dim3(1), dim3(1), 0, stream_, tabs(), id);
+    release(slots_[id].off, slots_[id].bytes);
+    slots_[id] = Slot{};
+    return rebuild_tiles();
+  }
+
+  int run(int rounds, int* active) override {
+    if (rounds < 0) return fail("run: rounds must be >= 0");
+    if (use_graph_) {
+      if (!graph_exec_) {
+        int rc = capture();
+        if (rc) return rc;
+      }
+      const int launches = (rounds + graph_rounds_ - 1) / graph_rounds_;
+      for (int k = 0; k < launches; ++k)
+        if (hipGraphLaunch(graph_exec_, stream_) != hipSuccess) return fail("graph launch failed");
+    } else {
+      for (int k = 0; k < rounds; ++k) launch_round();
+    }
+    if (hipGetLastError() != hipSuccess) return fail("round launch failed");
+    if (hipMemcpyAsync(h_meta_, d_meta_, sizeof(int) * 8, hipMemcpyDeviceToHost, stream_) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+      return fail("run: synchronisation failed");
+    if (active) *active = h_meta_[2];
+    return JX_OK;
+  }
+
+  int poll(int* done, double* time, int* cycle, int cap) override {
+    if (hipMemcpyAsync(h_ctrl_, d_ctrl_, sizeof(BatchCtrl) * max_inst_, hipMemcpyDeviceToHost,
+                       stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+      return fail("poll failed");
+    for (int k = 0; k < std::min(cap, max_inst_); ++k) {
+      const bool live = slots_[k].live;
+      if (done) done[k] = live ? (h_ctrl_[k].c.done ? 1 : 0) : -1;
+      if (time) time[k] = h_ctrl_[k].c.time;
+      if (cycle) cycle[k] = h_ctrl_[k].c.cycle;
+    }
+    return JX_OK;
+  }
+
+  int result(int id, void* U_out, double* time, double* dt, int* cycle) override {
+    if (id < 0 || id >= max_inst_ || !slots_[id].live) return fail("result: no such instance");
+    const jx_run_params& p = slots_[id].p;
+    if (U_out &&
+        hipMemcpyAsync(U_out, h_inst_[id].A, (size_t)p.nx * p.ny * p.nvar * sizeof(T),
+                       hipMemcpyDeviceToDevice, stream_) != hipSuccess)
+      return fail("result: copy failed");
+    if (hipMemcpyAsync(&h_ctrl_[id], &d_ctrl_[id], sizeof(BatchCtrl), hipMemcpyDeviceToHost,
+                       stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+      return fail("result: synchronisation failed");
+    if (time) *time = h_ctrl_[id].c.time;
+    if (dt) *dt = h_ctrl_[id].c.prev_dt;
+    if (cycle) *cycle = h_ctrl_[id].c.cycle;
+    return JX_OK;
+  }
+
+  int state_ptr(int id, void** p) override {
+    if (id < 0 || id >= max_inst_ || !slots_[id].live || !p) return fail("state_ptr: bad id");
+    *p = h_inst_[id].A;
+    return JX_OK;
+  }
+
+  int set_graph(int on, int rounds) override {
+    if (rounds < 1 || rounds > 64) return fail("graph rounds must be in [1, 64]");
+    use_graph_ = on != 0;
+    graph_rounds_ = rounds;
+    drop_graph();
+    return JX_OK;
+  }
+
+ private:
+  struct Slot {
+    bool live = false;
+    size_t off = 0, bytes = 0;
+    int rows = 32;
+    jx_run_params p{};
+  };
+
+  static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
+  static constexpr size_t smem() { return (size_t)4 * (3 * kBatchNV + 9) * 64 * sizeof(T); }
+
+  int fail(const std::string& m, int code = JX_EINVAL) {
+    err = m;
+    return code;
+  }
+
+  BatchTabs<T> tabs() const {
+    return BatchTabs<T>{d_inst_, d_ctrl_, d_tiles_, d_meta_, d_cnt_, d_partial_};
+  }
+
+  long long alloc(size_t need) {
+    need = align(need);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= need) {
+        const size_t off = it->first, sz = it->second;
+        free_.erase(it);
+        if (sz > need) free_[off + need] = sz - need;
+        return (long long)off;
+      }
+    }
+    return -1;
+  }
+
+  void release(size_t off, size_t sz) {
+    sz = align(sz);
+    auto it = free_.emplace(off, sz).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+  long long ntiles_total() const {
+    long long t = 0;
+    for (int k = 0; k < max_inst_; ++k)
+      if (slots_[k].live) {
+        const jx_run_params& p = slots_[k].p;
+        t += (long long)((p.nx + kTileCols - 1) / kTileCols) *
+             ((p.ny + slots_[k].rows - 1) / slots_[k].rows);
+      }
+    return t;
+  }
+
+  // Tiles of every live instance, largest instances first; uploads tiles, instance table
+  // and the meta counts (the graph reads them from device memory, so it stays valid).
+  int rebuild_tiles() {
+    std::vector<int> order;
+    int top = 0;
+    for (int k = 0; k < max_inst_; ++k)
+      if (slots_[k].live) {
+        order.push_back(k);
+        top = k + 1;
+      }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (long long)slots_[a].p.nx * slots_[a].p.ny > (long long)slots_[b].p.nx * slots_[b].p.ny;
+    });
+    h_tiles_.clear();
+    for (int k : order) {
+      const jx_run_params& p = slots_[k].p;
+      const int r = slots_[k].rows;
+      h_inst_[k].tile0 = (int)h_tiles_.size();
+      for (int y = 0; y < p.ny; y += r)
+        for (int x = 0; x < p.nx; x += kTileCols) h_tiles_.push_back(Tile{k, x, y, r});
+      h_inst_[k].ntiles = (int)h_tiles_.size() - h_inst_[k].tile0;
+    }
+    if ((long long)h_tiles_.size() > max_tiles_) return fail("tile list full");
+    int meta[2] = {top, (int)h_tiles_.size()};
+    if (!h_tiles_.empty() &&
+        hipMemcpyAsync(d_tiles_, h_tiles_.data(), sizeof(Tile) * h_tiles_.size(),
+                       hipMemcpyHostToDevice, stream_) != hipSuccess)
+      return fail("tile upload failed");
+    if (hipMemcpyAsync(d_inst_, h_inst_.data(), sizeof(BatchInst<T>) * max_inst_,
+                       hipMemcpyHostToDevice, stream_) != hipSuccess ||
+        hipMemcpyAsync(d_meta_, meta, sizeof(meta), hipMemcpyHostToDevice, stream_) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+      return fail("table upload failed");
+    return JX_OK;
+  }
+
+  void launch_round() {
+    const BatchTabs<T> tb = tabs();
+    const dim3 g(grid_), blk(kBatchBlock);
+    const dim3 gi((max_inst_ + 255) / 256);
+    hipLaunchKernelGGL(batch_control_kernel<T>, dim3(1), dim3(256), 0, stream_, tb);
+    hipLaunchKernelGGL(batch_render_kernel<T>, g, blk, 0, stream_, tb);
+    hipLaunchKernelGGL(batch_stage_kernel<T>, g, blk, smem(), stream_, tb);
+    hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 0);
+    hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 0);
+    hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 1);
+    hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 1);
+    hipLaunchKernelGGL(batch_timestep_kernel<T>, g, blk, 0, stream_, tb, -1);
+    hipLaunchKernelGGL(batch_advance_kernel<T>, dim3(1), dim3(256), 0, stream_, tb);
+  }
+
+  int capture() {
+    drop_graph();
+    if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail("graph capture could not start");
+    for (int k = 0; k < graph_rounds_; ++k) launch_round();
+    if (hipStreamEndCapture(stream_, &graph_) != hipSuccess || !graph_)
+      return fail("graph capture failed");
+    if (hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0) != hipSuccess)
+      return fail("graph instantiation failed");
+    return JX_OK;
+  }
+
+  void drop_graph() {
+    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+    if (graph_) hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+  }
+
+  int device_;
+  int max_inst_;
+  long long max_tiles_, pool_bytes_;
+  hipStream_t stream_ = nullptr;
+  int cus_ = 0, grid_ = 1;
+  BatchInst<T>* d_inst_ = nullptr;
+  BatchCtrl* d_ctrl_ = nullptr;
+  Tile* d_tiles_ = nullptr;
+  int* d_meta_ = nullptr;
+  unsigned* d_cnt_ = nullptr;
+  double* d_partial_ = nullptr;
+  void* d_pool_ = nullptr;
+  BatchCtrl* h_ctrl_ = nullptr;
+  int* h_meta_ = nullptr;
+  std::vector<BatchInst<T>> h_inst_;
+  std::vector<Slot> slots_;
+  std::vector<Tile> h_tiles_;
+  std::map<size_t, size_t> free_;
+  bool use_graph_ = true;
+  int graph_rounds_ = 6;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+};
+
+}  // namespace jx

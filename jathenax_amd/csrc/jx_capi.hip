@@ -11,7 +11,18 @@
 
 #include "../../include/jathenax_amd.h"
 #include "jx_aux.hpp"
+#include "jx_batch_host.hpp"
 #include "jx_stage.hpp"
+
+jx::BatchBase* jx_make_batch_f32(int device, int max_inst, long long max_tiles,
+                                 long long pool_bytes, std::string* err);
+jx::BatchBase* jx_make_batch_f64(int device, int max_inst, long long max_tiles,
+                                 long long pool_bytes, std::string* err);
+
+struct jx_batch {
+  jx::BatchBase* impl;
+  std::string err;
+};
 
 using jx::Ctrl;
 using jx::StageArgs;
@@ -814,6 +825,67 @@ int jx_conserved_sums(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void
   JX_HIP(c, hipStreamSynchronize(c->stream));
   std::memcpy(sums_out, c->host_sums, nvar * sizeof(double));
   return JX_OK;
+}
+
+// ---- batched ensemble engine ----------------------------------------------------------
+
+int jx_batch_create(int device, int dtype, int max_instances, long long max_tiles,
+                    long long pool_bytes, jx_batch** out) {
+  if (!out) return JX_EINVAL;
+  *out = nullptr;
+  if ((dtype != 0 && dtype != 1) || max_instances < 1 || max_tiles < 1 || pool_bytes < 256)
+    return JX_EINVAL;
+  std::string err;
+  jx::BatchBase* impl = dtype == 1
+      ? jx_make_batch_f32(device, max_instances, max_tiles, pool_bytes, &err)
+      : jx_make_batch_f64(device, max_instances, max_tiles, pool_bytes, &err);
+  if (!impl) {
+    std::fprintf(stderr, "jx_batch_create: %s\n", err.c_str());
+    return JX_EHIP;
+  }
+  *out = new jx_batch{impl, ""};
+  return JX_OK;
+}
+
+void jx_batch_destroy(jx_batch* b) {
+  if (!b) return;
+  delete b->impl;
+  delete b;
+}
+
+const char* jx_batch_last_error(const jx_batch* b) { return b ? b->impl->err.c_str() : "null"; }
+
+long long jx_batch_footprint(const jx_batch* b, const jx_run_params* p, int forced) {
+  return (b && p) ? b->impl->footprint(p, forced) : -1;
+}
+
+long long jx_batch_pool_free(const jx_batch* b) { return b ? b->impl->pool_free() : -1; }
+
+int jx_batch_add(jx_batch* b, const jx_run_params* p, const void* U0, const jx_forcing* f,
+                 int rows, int* id_out) {
+  return b ? b->impl->add(p, U0, f, rows, id_out) : JX_EINVAL;
+}
+
+int jx_batch_remove(jx_batch* b, int id) { return b ? b->impl->remove(id) : JX_EINVAL; }
+
+int jx_batch_run(jx_batch* b, int rounds, int* active_out) {
+  return b ? b->impl->run(rounds, active_out) : JX_EINVAL;
+}
+
+int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap) {
+  return b ? b->impl->poll(done, time, cycle, cap) : JX_EINVAL;
+}
+
+int jx_batch_result(jx_batch* b, int id, void* U_out, double* time, double* dt, int* cycle) {
+  return b ? b->impl->result(id, U_out, time, dt, cycle) : JX_EINVAL;
+}
+
+int jx_batch_state(jx_batch* b, int id, void** state_out) {
+  return b ? b->impl->state_ptr(id, state_out) : JX_EINVAL;
+}
+
+int jx_batch_set_graph(jx_batch* b, int use_graph, int rounds_per_graph) {
+  return b ? b->impl->set_graph(use_graph, rounds_per_graph) : JX_EINVAL;
 }
 
 }  // extern "C"

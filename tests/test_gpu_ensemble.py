@@ -1,0 +1,142 @@
+"""Batched ensemble engine: each instance equals its own single-grid run, and an instance's
+result does not depend on the rest of the batch (bitwise)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import jathenax_np as ox
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def jx():
+    import jathenax_amd
+    assert torch.cuda.is_available()
+    return jathenax_amd
+
+
+def _specs():
+    """A mixed set: shapes (square and not, ragged strips), operators, integrators,
+    scalars, viscosity/Smagorinsky/drag and one forced instance."""
+    from jathenax_amd import forcing as jf, problems
+    rng = np.random.default_rng(7)
+    out = []
+    combos = [((64, 48), "ppm4", "roe", "rk3", 3, {}),
+              ((100, 70), "wenoz", "hlle", "rk2", 4, dict(viscosity=1e-3, smagorinsky_cs=0.17)),
+              ((128, 128), "plm", "llf", "rk3", 4, dict(drag_rate=0.5)),
+              ((58, 131), "ppmx", "roe", "rk4", 3, dict(viscosity=2e-3)),
+              ((256, 64), "dc", "hlle", "rk3", 3, {}),
+              ((80, 80), "ppm4", "llf", "rk2", 4, dict(smagorinsky_cs=0.2, drag_rate=0.3))]
+    for (nx, ny), recon, rs, integ, nv, phys in combos:
+        if nv == 4:
+            U = problems.kelvin_helmholtz_reference(nx, ny)
+        else:
+            U = problems.smooth_reference(nx, ny)
+        U = U + 0.01 * rng.standard_normal(U.shape) * (np.arange(U.shape[-1]) > 0)
+        out.append(dict(U=U, dx=1 / nx, dy=1 / ny, integ=integ, kw=dict(reconstruct=recon,
+                                                                        rsolver=rs, **phys)))
+    # forced instance
+    nx, ny = 72, 60
+    modes = jf.cartesian_modes(1, 3, (-3, 3), (-3, 3))
+    coef = jf.modal_coefficients(modes, (1.0, 1.0), 1, 3, 2.0, 1.0, 1618, 0.75, 0.002, 40)
+    basis = jf.forcing_basis(modes, problems.cell_centres(nx), problems.cell_centres(ny),
+                             (1.0, 1.0))
+    out.append(dict(U=problems.at_rest(nx, ny), dx=1 / nx, dy=1 / ny, integ="rk2",
+                    kw=dict(reconstruct="wenoz", rsolver="hlle", viscosity=1.5e-3,
+                            smagorinsky_cs=0.17, drag_rate=0.3333335, forcing=(coef, basis),
+                            dt_update=0.002, tdriv_start=0.0, dedt=0.5)))
+    return out
+
+
+@pytest.mark.parametrize("graph", (True, False))
+def test_instances_match_single_grid_runs_fp64(jx, graph):
+    from jathenax_amd.ensemble import Ensemble
+    specs = _specs()
+    ens = Ensemble(dtype=torch.float64, max_instances=16, pool_bytes=1 << 28, graph=graph)
+    ids = []
+    for s in specs:
+        ids.append(ens.add(torch.tensor(s["U"], device="cuda"), s["dx"], s["dy"], 1.0, 1e-10,
+                           s["integ"], 0.4, 0.06, 10 ** 6, **s["kw"]))
+    rounds = ens.run_until_done(chunk=12)
+    assert rounds > 0
+    for iid, s in zip(ids, specs):
+        U, t, dt, cyc = ens.result(iid)
+        Ur, tr, dtr, cr = jx.evolve_isothermal(torch.tensor(s["U"], device="cuda"), s["dx"],
+                                               s["dy"], 1.0, 1e-10, s["integ"], 0.4, 0.06, 10 ** 6,
+                                               **s["kw"])
+        assert cyc == cr and t == tr, (iid, cyc, cr, t, tr)
+        if "forcing" in s["kw"]:
+            assert _rel(_np(U), _np(Ur)) <= 1e-12
+        else:
+            np.testing.assert_array_equal(_np(U), _np(Ur))
+
+
+def test_instance_alone_equals_instance_in_mix_fp32(jx):
+    from jathenax_amd.ensemble import Ensemble
+    specs = _specs()
+    target = specs[-1]   # the forced one: its kick sums must not see the other instances
+
+    def run(with_others):
+        ens = Ensemble(dtype=torch.float32, max_instances=16, pool_bytes=1 << 27)
+        if with_others:
+            for s in specs[:3]:
+                ens.add(torch.tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"],
+                        s["dy"], 1.0, 1e-10, s["integ"], 0.4, 0.05, 10 ** 6, **s["kw"])
+        iid = ens.add(torch.tensor(target["U"], dtype=torch.float32, device="cuda"),
+                      target["dx"], target["dy"], 1.0, 1e-10, target["integ"], 0.4, 0.05, 10 ** 6,
+                      **target["kw"])
+        if with_others:
+            for s in specs[3:-1]:
+                ens.add(torch.tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"],
+                        s["dy"], 1.0, 1e-10, s["integ"], 0.4, 0.05, 10 ** 6, **s["kw"])
+        ens.run_until_done(chunk=12)
+        return ens.result(iid)
+
+    a, b = run(False), run(True)
+    np.testing.assert_array_equal(_np(a[0]), _np(b[0]))
+    assert a[1:] == b[1:]
+
+
+def test_ensemble_against_oracle(jx, ref_vectors):
+    from jathenax_amd.ensemble import Ensemble
+    ens = Ensemble(dtype=torch.float64, max_instances=4, pool_bytes=1 << 26)
+    U0 = ref_vectors["run_smooth_U0"]
+    a = ens.add(torch.tensor(U0, device="cuda"), 1 / 64, 1 / 48, 1.0, 1e-10, "rk3", 0.4, 10.0, 100)
+    b = ens.add(torch.tensor(ref_vectors["run_kh_U0"], device="cuda"), 1 / 32, 1 / 32, 1.0, 1e-10,
+                "rk2", 0.4, 0.05, 1000, viscosity=1e-3, smagorinsky_cs=0.17, drag_rate=0.5,
+                reconstruct="wenoz", rsolver="hlle")
+    ens.run_until_done(chunk=12)
+    U, t, dt, cyc = ens.result(a)
+    assert cyc == 100 and _rel(_np(U), ref_vectors["run_smooth_n100_U"]) <= 1e-12
+    U, t, dt, cyc = ens.result(b)
+    assert _rel(_np(U), ref_vectors["run_kh_rk2_wenoz_hlle_U"]) <= 1e-12
+
+
+def test_remove_and_reuse_slots(jx):
+    from jathenax_amd import problems
+    from jathenax_amd.ensemble import Ensemble
+    ens = Ensemble(dtype=torch.float32, max_instances=3, pool_bytes=1 << 24)
+    U = torch.tensor(problems.smooth_reference(64, 64), dtype=torch.float32, device="cuda")
+    ids = [ens.add(U, 1 / 64, 1 / 64, 1.0, 1e-10, "rk3", 0.4, 0.02, 100) for _ in range(3)]
+    with pytest.raises(ValueError):
+        ens.add(U, 1 / 64, 1 / 64, 1.0, 1e-10, "rk3", 0.4, 0.02, 100)
+    ens.run_until_done(chunk=6)
+    ref = ens.result(ids[0])
+    ens.remove(ids[1])
+    again = ens.add(U, 1 / 64, 1 / 64, 1.0, 1e-10, "rk3", 0.4, 0.02, 100)
+    assert again == ids[1]
+    ens.run_until_done(chunk=6)
+    got = ens.result(again)
+    np.testing.assert_array_equal(_np(got[0]), _np(ref[0]))
+    with pytest.raises(KeyError):
+        ens.add(U, 1 / 64, 1 / 64, 1.0, 1e-10, "rk3", 0.4, 0.02, 100, reconstruct="weno9")
