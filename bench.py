@@ -42,6 +42,12 @@ def parse():
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 512 MiB device-to-device copy (PMC byte calibration)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="instances in the batched-ensemble leg (0 = skip)")
+    ap.add_argument("--serve", type=int, default=16,
+                    help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, 1/s")
     return ap.parse_args()
 
 
@@ -67,6 +73,60 @@ def gpu_leg(jx, problems, n, dtype, steps, warmup, block_x, rows):
     finite = bool(torch.isfinite(U).all())
     del U
     return dict(wall=wall, launches=launches, kernel_ms=kernel_ms, finite=finite)
+
+
+def ensemble_leg(n_inst):
+    """Batched ensemble: n_inst seeded instances (64..512 per axis, KH / driven turbulence,
+    mixed operators, RK2/RK3, FP32) in one engine; latency = time from the start of the
+    run to each instance's completion, observed between graph replays."""
+    import torch
+    from jathenax_amd import workloads
+    from jathenax_amd.ensemble import Ensemble
+    specs = workloads.ensemble_mix(n=n_inst)
+    ens = Ensemble(dtype=torch.float32, max_instances=n_inst, pool_bytes=8 << 30)
+    ids = [ens.add(torch.as_tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"], s["dy"],
+                   s["cs"], s["dfloor"], s["integrator"], s["cfl"], s["tlim"], s["nlim"],
+                   **s["kw"]) for s in specs]
+    torch.cuda.synchronize()
+    finish = {}
+    t0 = time.perf_counter()
+    rounds = 0
+    while len(finish) < len(ids):
+        ens.run(6)
+        rounds += 6
+        now = time.perf_counter() - t0
+        done, _, _ = ens.poll()
+        for k, iid in enumerate(ids):
+            if k not in finish and done[iid] == 1:
+                finish[k] = now
+    wall = time.perf_counter() - t0
+    _, _, cycles = ens.poll()
+    work = sum(workloads.zone_stages(s, int(cycles[iid])) for s, iid in zip(specs, ids))
+    lat = np.asarray([finish[k] for k in range(len(ids))])
+    zones = sum(s["U"].shape[0] * s["U"].shape[1] for s in specs)
+    return {"instances": len(ids), "total_zones": int(zones), "rounds": rounds,
+            "wall_s": wall, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "zone_updates_per_s": work / wall,
+            "workload": "ensemble_mix seed 20261203: sizes log-uniform 64..512 per axis, KH "
+                        "(tlim 0.1) / driven turbulence (tlim 0.05), recon plm/ppm4/wenoz, "
+                        "rsolver llf/hlle/roe, rk2/rk3, FP32, one batched engine"}
+
+
+def serving_leg(n_req, rate):
+    """Continuous-batching server: Poisson arrivals, grids 128^2..2048^2, tlim U[0.1, 2]."""
+    from jathenax_amd import workloads
+    from jathenax_amd.serving import Request, Server
+    reqs = [Request(rid=k, spec=spec, arrival=t) for k, (t, spec) in
+            enumerate(workloads.serving_requests(n=n_req, rate=rate))]
+    srv = Server(pool_bytes=64 << 30, max_instances=max(8, n_req), keep_results=False)
+    t0 = time.perf_counter()
+    out = srv.serve(reqs)
+    wall = time.perf_counter() - t0
+    work = sum(workloads.zone_stages(r.spec, r.info.get("cycles", 0)) for r in out)
+    rep = Server.report(out, wall, work)
+    rep["workload"] = (f"{n_req} requests, Poisson rate {rate}/s, square grids log-uniform "
+                       "128..2048, tlim U[0.1, 2], KH / driven turbulence, PPM4+Roe RK3, FP32")
+    return rep
 
 
 def cpu_leg(problems, n):
@@ -112,6 +172,16 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    tiles = {}
+    if not a.no_tune and not (a.block_x or a.rows):
+        from jathenax_amd import tuning
+        for dt_name, tdt in (("f32", torch.float32), ("f64", torch.float64)):
+            U0 = torch.tensor(problems.smooth_wave(n, n), dtype=tdt, device="cuda")
+            choice, _ = tuning.tune_single(U0, 1.0 / n, 1.0 / n, 1.0, 1e-10, "rk3", steps=2,
+                                           repeats=2)
+            tiles[dt_name] = list(choice)
+            del U0
+        print(f"# bench: tuned tiles {tiles}", file=sys.stderr, flush=True)
     print(f"# bench: {n}x{n} {a.dtype} steps={K} warmup={a.warmup}", file=sys.stderr, flush=True)
     main_leg = gpu_leg(jx, problems, n, a.dtype, K, a.warmup, a.block_x, a.rows)
     zu = n * n * stages * K / main_leg["wall"]
@@ -127,7 +197,7 @@ def main():
         "config": {"workload": f"{n}x{n} smooth wave, PPM4+Roe, RK3, nvar=3, CFL 0.4, "
                                f"device-resident time loop", "nx": n, "ny": n,
                    "reconstruct": "ppm4", "rsolver": "roe", "integrator": "rk3",
-                   "stages_per_step": stages},
+                   "stages_per_step": stages, "tiles": tiles or "auto"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(n, a.dtype),
                      "kernel": "jx::stage_kernel (fused RK stage)",
@@ -142,6 +212,12 @@ def main():
         out["fp64"] = {"value": zu64, "ms_per_step": f64["wall"] / K * 1e3,
                        "avg_launch_ms": f64["kernel_ms"] / max(1, f64["launches"]),
                        "fp32_over_fp64": zu / zu64}
+    if a.ensemble > 0:
+        print(f"# bench: ensemble leg ({a.ensemble} instances)", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_leg(a.ensemble)
+    if a.serve > 0:
+        print(f"# bench: serving leg ({a.serve} requests)", file=sys.stderr, flush=True)
+        out["serving"] = serving_leg(a.serve, a.serve_rate)
     if not a.no_cpu_baseline:
         print("# bench: cpu baseline", file=sys.stderr, flush=True)
         rate, sec = cpu_leg(problems, a.cpu_n)

@@ -51,6 +51,7 @@ class Context:
                                      ctypes.c_void_p(stream_ptr)), None, "jx_ctx_create failed")
         self.handle = h
         self.device_index = device_index
+        self.pinned = False
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -72,6 +73,11 @@ class Context:
         return n.value, ms.value
 
     def set_tiles(self, block_x=0, rows=0):
+        """Pin the stage kernel's tile shape for this context; (0, 0) unpins (tuned/auto)."""
+        self.pinned = (int(block_x), int(rows)) != (0, 0)
+        self._apply_tiles(block_x, rows)
+
+    def _apply_tiles(self, block_x, rows):
         self.check(_lib.load().jx_set_tiles(self.handle, int(block_x), int(rows)), "set_tiles")
 
 

@@ -100,6 +100,9 @@ class Ensemble:
                              sin_y=tabs[3].data_ptr(), n_updates=coef.shape[0],
                              n_modes=coef.shape[1])
             fptr = ctypes.byref(f)
+        if not rows:
+            from . import tuning
+            rows = tuning.batch_rows(p.nx, p.ny)
         # The batch runs on its own stream: make the caller's writes visible first.
         torch.cuda.current_stream(self.device).synchronize()
         iid = ctypes.c_int()

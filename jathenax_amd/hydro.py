@@ -118,6 +118,8 @@ def isothermal_step(U, dt, dx, dy, cs, dfloor, integrator, *,
             raise ValueError("raw_force must have shape (ny, nx, 2)")
     out = torch.empty_like(U)
     ctx = context_for(U)
+    from . import tuning
+    tuning.apply_single(ctx, p)
     ctx.check(_lib.load().jx_isothermal_step(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
                                              float(dt), ptr(raw), float(dedt)),
               "isothermal_step")

@@ -14,9 +14,14 @@ import math
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, tuning
 from .device import as_device, context_for, dtype_id, ptr
 from .hydro import run_params
+
+
+def _evolve_raw(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, **kw):
+    """evolve_isothermal with the context's current tiles (used by the tile tuner)."""
+    return _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, apply_table=False, **kw)
 
 
 def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
@@ -37,6 +42,15 @@ def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
             smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
             dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt, reconstruct=reconstruct,
             rsolver=rsolver, predict_stress=predict_stress, sgs_weights=sgs_weights)
+    return _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, viscosity=viscosity,
+                   smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
+                   dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt,
+                   reconstruct=reconstruct, rsolver=rsolver, apply_table=True)
+
+
+def _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, viscosity=0.0,
+            smagorinsky_cs=0.0, drag_rate=0.0, forcing=None, dt_update=1.0, tdriv_start=0.0,
+            dedt=0.0, reconstruct="ppm4", rsolver="roe", apply_table=True):
     U = as_device(U)
     p = run_params(U, dx, dy, cs, dfloor, integrator, reconstruct=reconstruct, rsolver=rsolver,
                    viscosity=viscosity, smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate,
@@ -61,6 +75,8 @@ def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
     out = torch.empty_like(U)
     t, dt, cyc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
     ctx = context_for(U)
+    if apply_table:
+        tuning.apply_single(ctx, p)
     ctx.check(_lib.load().jx_evolve_isothermal(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
                                                fptr, ctypes.byref(t), ctypes.byref(dt),
                                                ctypes.byref(cyc)), "evolve_isothermal")

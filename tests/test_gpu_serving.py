@@ -1,0 +1,59 @@
+"""Continuous-batching server: results in a mix equal results alone; admission by HBM
+footprint; poisoned requests are evicted without disturbing the others."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n=6, seed=3):
+    from jathenax_amd import workloads
+    from jathenax_amd.serving import Request
+    reqs = workloads.serving_requests(n=n, rate=200.0, seed=seed, tmin=0.01, tmax=0.03,
+                                      size_lo=64, size_hi=160, dt_update=1e-3)
+    return [Request(rid=k, spec=spec, arrival=t) for k, (t, spec) in enumerate(reqs)]
+
+
+def test_served_in_mix_equals_served_alone():
+    from jathenax_amd.serving import Request, Server
+    mix = Server(pool_bytes=1 << 28, max_instances=16).serve(_requests())
+    assert all(r.status == "done" for r in mix), [(r.rid, r.status, r.info) for r in mix]
+    for r in mix:
+        alone = Server(pool_bytes=1 << 28, max_instances=2).serve(
+            [Request(rid=r.rid, spec=r.spec, arrival=0.0)])[0]
+        assert alone.status == "done"
+        np.testing.assert_array_equal(alone.result[0].cpu().numpy(), r.result[0].cpu().numpy())
+        assert alone.result[1:] == r.result[1:]
+
+
+def test_admission_respects_the_hbm_budget():
+    from jathenax_amd.serving import Server
+    reqs = _requests(n=5, seed=9)
+    srv = Server(pool_bytes=1 << 28, max_instances=16)
+    biggest = max(srv._footprint(r.spec) for r in reqs)
+    srv = Server(pool_bytes=2 * biggest + 4096, max_instances=16)
+    peak = []
+    orig = srv._admit
+
+    def admit():
+        orig()
+        peak.append(len(srv.running))
+    srv._admit = admit
+    out = srv.serve(reqs)
+    assert all(r.status == "done" for r in out)
+    assert max(peak) <= 2
+
+
+def test_poisoned_request_is_evicted():
+    from jathenax_amd.serving import Server
+    reqs = _requests(n=4, seed=5)
+    reqs[1].spec = dict(reqs[1].spec)
+    U = np.array(reqs[1].spec["U"], copy=True)
+    U[3, 3, 1] = np.nan
+    reqs[1].spec["U"] = U
+    reqs[1].max_rounds = 600
+    out = {r.rid: r for r in Server(pool_bytes=1 << 28, max_instances=8).serve(reqs)}
+    assert out[1].status == "error"
+    assert all(out[k].status == "done" for k in (0, 2, 3))
