@@ -145,15 +145,17 @@ int jx_modal_coefficients(const int* modes, int n_modes, double lx, double ly, i
  * stream) into the pool, so the caller may free U0 afterwards.  Results are bitwise
  * independent of which other instances share the batch. */
 typedef struct jx_batch jx_batch;
-int jx_batch_create(int device, int dtype, int max_instances, long long max_tiles,
-                    long long pool_bytes, jx_batch** out);
+int jx_batch_create(int device, int dtype, int max_instances, long long pool_bytes,
+                    jx_batch** out);
 void jx_batch_destroy(jx_batch* b);
 const char* jx_batch_last_error(const jx_batch* b);
 /* HBM bytes an instance of these parameters takes from the pool (admission control). */
 long long jx_batch_footprint(const jx_batch* b, const jx_run_params* p, int forced);
 long long jx_batch_pool_free(const jx_batch* b);
 /* Register an instance (tlim, nlim, cfl and the forcing schedule come from p); `rows`
- * is its tile height (0 = default).  forcing may be NULL. */
+ * caps its tile height (rounded down to a multiple of 8, at least 8; 0 = 32): each round
+ * the engine lowers the height of every tile when the active work is too small to fill
+ * the GPU.  forcing may be NULL. */
 int jx_batch_add(jx_batch* b, const jx_run_params* p, const void* U0, const jx_forcing* f,
                  int rows, int* id_out);
 int jx_batch_remove(jx_batch* b, int id);
@@ -165,6 +167,9 @@ int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap);
 int jx_batch_result(jx_batch* b, int id, void* U_out, double* time, double* dt, int* cycle);
 int jx_batch_state(jx_batch* b, int id, void** state_out);
 int jx_batch_set_graph(jx_batch* b, int use_graph, int rounds_per_graph);
+/* Tiles per resident wave that each round's tile heights aim for (default 2); 0 keeps
+ * every instance at its own cap.  Results do not depend on it. */
+int jx_batch_set_fill(jx_batch* b, int fill);
 
 #ifdef __cplusplus
 }

@@ -66,7 +66,7 @@ SIGNATURES = {
     "jx_modal_coefficients": (c_int, [POINTER(c_int), c_int, c_double, c_double, c_int, c_int,
                                       c_double, c_double, c_longlong, c_double, c_double, c_int,
                                       POINTER(c_double)]),
-    "jx_batch_create": (c_int, [c_int, c_int, c_int, c_longlong, c_longlong, POINTER(c_void_p)]),
+    "jx_batch_create": (c_int, [c_int, c_int, c_int, c_longlong, POINTER(c_void_p)]),
     "jx_batch_destroy": (None, [c_void_p]),
     "jx_batch_last_error": (ctypes.c_char_p, [c_void_p]),
     "jx_batch_footprint": (c_longlong, [c_void_p, POINTER(RunParams), c_int]),
@@ -80,6 +80,7 @@ SIGNATURES = {
                                 POINTER(c_int)]),
     "jx_batch_state": (c_int, [c_void_p, c_int, POINTER(c_void_p)]),
     "jx_batch_set_graph": (c_int, [c_void_p, c_int, c_int]),
+    "jx_batch_set_fill": (c_int, [c_void_p, c_int]),
 }
 
 _lib = None

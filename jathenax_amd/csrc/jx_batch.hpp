@@ -2,20 +2,28 @@
 // operators advanced together by persistent wave-worker kernels.
 //
 // Work unit.  A tile is a 64-column strip (58 output columns plus the 3+3 halo) of one
-// instance over `rows` rows.  Each 64-lane wave is an independent worker: it takes the
-// next tile from a device counter, runs the instance's current RK stage on it with the
+// instance over a band of rows.  Each 64-lane wave is an independent worker: it takes the
+// next tile of the round's work list, runs the instance's current RK stage on it with the
 // same column march as the single-grid kernel (x exchanges through the wave's own LDS
 // slice, so no block barrier couples workers), and moves on.  Reconstruction, Riemann
-// solver, viscosity and the number of variables are per-instance runtime choices, uniform
-// across a wave.
+// solver and viscosity are per-instance runtime choices, uniform across a wave; the
+// number of variables selects one of two compiled tile bodies.
+//
+// Tiles are not stored: tile q of an instance is strip q % nstrips of row band
+// q / nstrips.  The band height is chosen every round by the control kernel: the largest
+// multiple of 8 rows (at most the instance's cap) that still gives every resident wave
+// about `fill` tiles, so a round with little work is cut into small tiles that keep the
+// whole GPU busy, and a round with much work uses tall ones that amortise the 6-row
+// warm-up of the column march.  Sums over an instance (the kick moments) are taken per
+// 8-row group of a strip and then over groups in a fixed order, so they do not depend on
+// the band height, and an instance's result does not depend on the rest of the batch.
 //
 // Rounds.  One round advances every active instance by one RK stage of its own schedule:
-// control (begin-step for instances at stage 0) -> forcing render -> stage -> kick
-// (mean, moments, scale, apply; forced instances at their final stage) -> Smagorinsky
-// timestep -> advance.  All control lives in device memory, so a round is a fixed
-// sequence of launches that is captured once in a hipGraph and replayed.  Every
-// reduction is per instance and in a fixed order, so an instance's result does not
-// depend on which other instances share the batch.
+// control (advance the stage, begin-step for instances at stage 0, band heights, work
+// lists) -> stage (with the forcing render of stage 0) -> kick (mean, moments, scale,
+// apply; forced instances at their final stage) -> Smagorinsky timestep.  All control
+// lives in device memory, so a round is a fixed sequence of launches that is captured
+// once in a hipGraph and replayed.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,10 +39,13 @@ template <typename T> struct BatchInst {
   T* A;                  // state between steps (floored after every step)
   T* B; T* C; T* S;      // stage registers
   T* raw;                // rendered raw force (forced instances)
+  double* part;          // kick sums, 3 doubles per (strip, 8-row group), strip-major
   const T* coef; const T* cx; const T* sx; const T* cy; const T* sy;
   int nx, ny, nv, recon, rs, nstages;
   int visc_on, smag_on, drag_on, forced, n_updates, n_modes;
-  int tile0, ntiles;
+  int nstrips, ngroups;  // 58-column strips, 8-row groups
+  int rows_cap;          // largest band height (multiple of 8)
+  int pad_;
   T dx, dy, cs, cs2, inv_cs, dfloor, eps, four_dx, four_dy, visc, smag_c, drag;
   T inv_dx, inv_dy, inv_four_dx, inv_four_dy;
   LoopArgs loop;
@@ -45,21 +56,73 @@ struct BatchCtrl {
   int stage;    // RK stage this round executes
   int active;   // takes part in this round
   int live;     // slot holds an instance
-  int pad;
+  int rows;     // band height of this round's tiles
+  int ntiles;   // tiles of this round
+  int pad[3];
 };
 
-struct Tile { int inst, x0, y0, rows; };
+struct Tile { int x0, y0, rows, strip; };
 
-enum { CNT_RENDER = 0, CNT_STAGE, CNT_MOM, CNT_APPLY, CNT_TSTEP, CNT_TOTAL };
+constexpr int kGroupRows = 8;
+
+enum { CNT_STAGE = 0, CNT_MOM, CNT_APPLY, CNT_TSTEP, CNT_TOTAL };
+
+// Per-round work lists, built by the control kernel: the instances that need a kernel
+// this round and the prefix sums of their tile counts, so worker waves only visit the
+// tiles that have work.  Layout of list L at lists + L * list_stride(cap):
+//   [0] entries n, [1] total tiles, [2 .. 2+n] prefix (n+1 values), then n instance ids.
+enum { WL_STAGE = 0, WL_KICK, WL_SMAG, WL_TOTAL };
+
+__host__ __device__ inline int list_stride(int cap) { return 2 + (cap + 1) + cap; }
 
 template <typename T> struct BatchTabs {
   BatchInst<T>* inst;
   BatchCtrl* ctrl;
-  const Tile* tiles;
-  int* meta;              // [0] instance slots in use, [1] tiles, [2] active count
+  int* meta;              // [0] instance slots in use, [2] active count
   unsigned* counters;     // per-round work counters
-  double* partial;        // 3 doubles per tile
+  int* lists;             // WL_TOTAL work lists
+  int* flags;             // per-slot work bits of this round
+  int cap;                // instance slots
+  int nwaves;             // resident waves of a worker grid
+  int fill;               // tiles per wave the band heights aim for
 };
+
+// k-th tile of work list L (k < total): instance slot s and its tile q.
+template <typename T>
+__device__ __forceinline__ void list_item(const BatchTabs<T>& b, const int* L, int k, int& s,
+                                          int& q) {
+  const int n = L[0];
+  const int* pref = L + 2;
+  const int* ids = pref + (b.cap + 1);
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pref[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  s = ids[lo];
+  q = k - pref[lo];
+}
+
+// Tile q of an instance at band height `rows`: strip q % nstrips of band q / nstrips.
+template <typename T>
+__device__ __forceinline__ Tile tile_of(const BatchInst<T>& I, int rows, int q) {
+  Tile t;
+  t.strip = q % I.nstrips;
+  t.x0 = t.strip * kTileCols;
+  t.y0 = (q / I.nstrips) * rows;
+  t.rows = min(rows, I.ny - t.y0);
+  return t;
+}
+
+// Fixed-order butterfly sum over the wave of three doubles.
+__device__ __forceinline__ void wave_sum3(double& a0, double& a1, double& a2) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off);
+    a1 += __shfl_xor(a1, off);
+    a2 += __shfl_xor(a2, off);
+  }
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -74,6 +137,17 @@ __device__ __forceinline__ int wave_next(unsigned* counter) {
   const unsigned inc = ((threadIdx.x & 63) == 0) ? 1u : 0u;
   const unsigned old = atomicAdd(counter, inc);
   return __builtin_amdgcn_readfirstlane((int)old);
+}
+
+// Work distribution of the persistent kernels: wave w of the grid takes item w first, then
+// items nwaves + (counter value) until the list is exhausted.  A kernel whose list is
+// shorter than the grid issues no atomic at all, and a long list costs one atomic per item
+// beyond the first wave of items.
+__device__ __forceinline__ int wave_first() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+}
+__device__ __forceinline__ int wave_then(unsigned* counter) {
+  return (int)(gridDim.x * (blockDim.x >> 6)) + wave_next(counter);
 }
 
 // RK rows (hydro.py:20-36) by number of stages and stage index.
@@ -131,20 +205,19 @@ __device__ __forceinline__ T over_rt(T x, T d, T inv) {
   else return x / d;
 }
 
-// One RK stage of one instance over one tile, executed by one wave.  Same arithmetic as
-// stage_kernel; `lds` is this wave's slice (21 columns of 64 values).
-template <typename T>
-__device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, const Tile& tl,
-                                 int k_tile, T* lds, double* partial) {
-  constexpr int NV = kBatchNV;
+// One RK stage of one instance (NV variables) over one tile, executed by one wave.  Same
+// arithmetic as stage_kernel; `lds` is this wave's slice (3 * kBatchNV + 9 rows of 64).
+template <typename T, int NV>
+__device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtrl& C,
+                                                 const Tile& tl, T* lds) {
   constexpr int R = 5;
   const int t = threadIdx.x & 63;
   T* sW = lds;             // [NV][64]
   T* sH = sW + NV * 64;    // [NV][64]
   T* sF = sH + NV * 64;    // [NV][64]
   T* sV = sF + NV * 64;    // [3][3][64]
-  const int nx = I.nx, ny = I.ny, nv = I.nv;
-  const bool nv4 = nv > 3;
+  const int nx = I.nx, ny = I.ny;
+  constexpr int nv = NV;
   const int x0 = tl.x0;
   int xc = (x0 + t - 3) % nx;
   if (xc < 0) xc += nx;
@@ -205,7 +278,7 @@ __device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, cons
     const T* src = u_in + ((size_t)yn * nx + xc) * nv;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
-      if (v < 3 || nv4) pf[v] = src[v];
+      pf[v] = src[v];
   }
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
   unsigned long long kmin = ~0ull;
@@ -223,7 +296,7 @@ __device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, cons
       const T* src = u_in + ((size_t)yn * nx + xc) * nv;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        if (v < 3 || nv4) pf[v] = src[v];
+        pf[v] = src[v];
     }
     T svr[NV];
 #pragma unroll
@@ -232,7 +305,7 @@ __device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, cons
       const T* sv = u_sv + ((size_t)(r - 3) * nx + xc) * nv;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        if (v < 3 || nv4) svr[v] = sv[v];
+        svr[v] = sv[v];
     }
     {
       const T rho = (u[S0][0] < dfloor) ? dfloor : u[S0][0];
@@ -345,7 +418,7 @@ __device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, cons
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
             us[v] = us[v] + T(delta) * u[S3][v];
-            if (v < 3 || nv4) s_out[cell * nv + v] = us[v];
+            s_out[cell * nv + v] = us[v];
           }
         }
 #pragma unroll
@@ -380,7 +453,19 @@ __device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, cons
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-          if (v < 3 || nv4) u_out[cell * nv + v] = un[v];
+          u_out[cell * nv + v] = un[v];
+      }
+      // Kick sums close at the end of every 8-row group (bands start on group bounds).
+      const int j = r - 3;
+      if (kick && (((j & (kGroupRows - 1)) == kGroupRows - 1) || j == yend - 1)) {
+        wave_sum3(acc0, acc1, acc2);
+        if (t == 0) {
+          double* g = I.part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
+          g[0] = acc0;
+          g[1] = acc1;
+          g[2] = acc2;
+        }
+        acc0 = acc1 = acc2 = 0.0;
       }
     }
     if (i >= 5) {
@@ -396,72 +481,146 @@ __device__ void batch_stage_tile(const BatchInst<T>& I, const BatchCtrl& C, cons
     if (i0 + 3 < nit) body(IC<3>{}, i0 + 3);
     if (i0 + 4 < nit) body(IC<4>{}, i0 + 4);
   }
-  // Wave reductions (fixed butterfly): kick moments per tile, or the CFL key.
-  for (int off = 32; off > 0; off >>= 1) {
-    acc0 += __shfl_xor(acc0, off);
-    acc1 += __shfl_xor(acc1, off);
-    acc2 += __shfl_xor(acc2, off);
-    const unsigned long long o = __shfl_xor(kmin, off);
-    kmin = o < kmin ? o : kmin;
-  }
-  if (t == 0) {
-    if (kick) {
-      partial[3 * (size_t)k_tile + 0] = acc0;
-      partial[3 * (size_t)k_tile + 1] = acc1;
-      partial[3 * (size_t)k_tile + 2] = acc2;
-    } else if (want_key) {
-      atomicMin(const_cast<unsigned long long*>(&C.c.dtkey), kmin);
+  if (want_key) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(kmin, off);
+      kmin = o < kmin ? o : kmin;
     }
+    if (t == 0) atomicMin(&C.c.dtkey, kmin);
   }
 }
+
+template <typename T>
+__device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C,
+                                                  const Tile& tl);
 
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_stage_kernel(BatchTabs<T> b) {
   extern __shared__ double bsm[];
   T* lds = reinterpret_cast<T*>(bsm) + (threadIdx.x >> 6) * (3 * kBatchNV + 9) * 64;
-  const int ntiles = b.meta[1];
-  for (;;) {
-    const int kt = wave_next(&b.counters[CNT_STAGE]);
-    if (kt >= ntiles) break;
-    const Tile tl = b.tiles[kt];
-    const BatchCtrl& C = b.ctrl[tl.inst];
-    if (C.active) batch_stage_tile<T>(b.inst[tl.inst], C, tl, kt, lds, b.partial);
+  const int* L = b.lists + WL_STAGE * list_stride(b.cap);
+  const int total = L[1];
+  for (int k = wave_first(); k < total; k = wave_then(&b.counters[CNT_STAGE])) {
+    int s, q;
+    list_item(b, L, k, s, q);
+    const BatchInst<T>& I = b.inst[s];
+    BatchCtrl& C = b.ctrl[s];
+    const Tile tl = tile_of(I, C.rows, q);
+    if (I.nv == 4) batch_stage_tile<T, 4>(I, C, tl, lds);
+    else batch_stage_tile<T, 3>(I, C, tl, lds);
+    // A step that opened a new forcing row renders the raw force of this tile (read by
+    // the kick at the step's final stage).
+    if (C.stage == 0 && I.forced && C.c.render) batch_render_tile<T>(I, C, tl);
   }
 }
 
-// Control: commit/open steps for instances at stage 0, decide who runs, reset counters.
+// Control, one block of kCtrlBlock threads: advance the stage of last round's workers,
+// open steps (begin-step) for instances at stage 0, pick this round's band height, and
+// build the work lists (slot order, block-wide scans in LDS) and reset the counters.
+constexpr int kCtrlBlock = 256;
+
 template <typename T>
-__global__ void batch_control_kernel(BatchTabs<T> b) {
-  if (threadIdx.x < CNT_TOTAL) b.counters[threadIdx.x] = 0u;
-  __shared__ int active;
-  if (threadIdx.x == 0) active = 0;
-  __syncthreads();
+__global__ void __launch_bounds__(kCtrlBlock) batch_control_kernel(BatchTabs<T> b) {
+  __shared__ int sc[WL_TOTAL][2][kCtrlBlock];
+  __shared__ int base[WL_TOTAL][2];
+  __shared__ long long red[kCtrlBlock];
+  const int tid = threadIdx.x;
+  if (tid < CNT_TOTAL) b.counters[tid] = 0u;
+  if (tid < WL_TOTAL) base[tid][0] = base[tid][1] = 0;
   const int n = b.meta[0];
-  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+  long long work = 0;   // strip-rows of this round
+  for (int s = tid; s < n; s += kCtrlBlock) {
     BatchCtrl& C = b.ctrl[s];
+    const BatchInst<T>& I = b.inst[s];
+    int f = 0;
     if (C.live) {
-      if (C.stage == 0) begin_step(&C.c, b.inst[s].loop);
+      if (C.active) C.stage = (C.stage + 1) % I.nstages;
+      if (C.stage == 0) begin_step(&C.c, I.loop);
       C.active = !C.c.done;
-      if (C.active) atomicAdd(&active, 1);
+      if (C.active) {
+        const bool last = C.stage == I.nstages - 1;
+        f = (1 << WL_STAGE) | ((last && I.forced) ? 1 << WL_KICK : 0) |
+            ((last && I.smag_on) ? 1 << WL_SMAG : 0);
+        work += (long long)I.nstrips * I.ny;
+      }
     } else {
       C.active = 0;
     }
+    b.flags[s] = f;
   }
+  red[tid] = work;
   __syncthreads();
-  if (threadIdx.x == 0) b.meta[2] = active;
-}
+  for (int off = kCtrlBlock / 2; off > 0; off >>= 1) {
+    if (tid < off) red[tid] += red[tid + off];
+    __syncthreads();
+  }
+  const long long target = (long long)b.nwaves * b.fill;
+  long long band = target > 0 ? red[0] / target : (1 << 20);   // fill 0: every cap
+  band = band < (1 << 20) ? band : (1 << 20);
+  const int rows_round = max(kGroupRows, (int)band / kGroupRows * kGroupRows);
 
-template <typename T>
-__global__ void batch_advance_kernel(BatchTabs<T> b) {
-  const int n = b.meta[0];
-  for (int s = threadIdx.x; s < n; s += blockDim.x) {
-    BatchCtrl& C = b.ctrl[s];
-    if (C.live && C.active) C.stage = (C.stage + 1) % b.inst[s].nstages;
+  for (int c0 = 0; c0 < n; c0 += kCtrlBlock) {
+    const int s = c0 + tid;
+    int f = 0, nt = 0;
+    if (s < n) {
+      f = b.flags[s];
+      if (f) {
+        BatchCtrl& C = b.ctrl[s];
+        const BatchInst<T>& I = b.inst[s];
+        const int rows = min(I.rows_cap, rows_round);
+        nt = I.nstrips * ((I.ny + rows - 1) / rows);
+        C.rows = rows;
+        C.ntiles = nt;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < WL_TOTAL; ++l) {
+      const int on = (f >> l) & 1;
+      sc[l][0][tid] = on;
+      sc[l][1][tid] = on ? nt : 0;
+    }
+    __syncthreads();
+    for (int off = 1; off < kCtrlBlock; off <<= 1) {   // inclusive Hillis-Steele scans
+      int v[WL_TOTAL][2];
+#pragma unroll
+      for (int l = 0; l < WL_TOTAL; ++l)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) v[l][j] = tid >= off ? sc[l][j][tid - off] : 0;
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < WL_TOTAL; ++l)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) sc[l][j][tid] += v[l][j];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int l = 0; l < WL_TOTAL; ++l) {
+      if ((f >> l) & 1) {
+        int* L = b.lists + l * list_stride(b.cap);
+        const int k = base[l][0] + sc[l][0][tid] - 1;
+        L[2 + k] = base[l][1] + sc[l][1][tid] - nt;          // prefix
+        L[2 + (b.cap + 1) + k] = s;                          // instance id
+      }
+    }
+    __syncthreads();
+    if (tid < WL_TOTAL) {
+      base[tid][0] += sc[tid][0][kCtrlBlock - 1];
+      base[tid][1] += sc[tid][1][kCtrlBlock - 1];
+    }
+    __syncthreads();
+  }
+  if (tid < WL_TOTAL) {
+    int* L = b.lists + tid * list_stride(b.cap);
+    L[0] = base[tid][0];
+    L[1] = base[tid][1];
+    L[2 + base[tid][0]] = base[tid][1];
+    if (tid == WL_STAGE) b.meta[2] = base[tid][0];
   }
 }
 
 template <typename T>
-__device__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C, const Tile& tl) {
+__device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C,
+                                                  const Tile& tl) {
   const int t = threadIdx.x & 63;
   const int row = C.c.update;
   const bool fill = (row < 0) || (row >= I.n_updates);
@@ -488,58 +647,48 @@ __device__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C, con
   }
 }
 
-// Raw-force render on the tiles of instances that opened a step with a new OU row.
-template <typename T>
-__global__ void __launch_bounds__(kBatchBlock) batch_render_kernel(BatchTabs<T> b) {
-  const int ntiles = b.meta[1];
-  for (;;) {
-    const int kt = wave_next(&b.counters[CNT_RENDER]);
-    if (kt >= ntiles) break;
-    const Tile tl = b.tiles[kt];
-    const BatchCtrl& C = b.ctrl[tl.inst];
-    const BatchInst<T>& I = b.inst[tl.inst];
-    if (C.active && C.stage == 0 && I.forced && C.c.render) batch_render_tile<T>(I, C, tl);
-  }
-}
-
 __device__ __forceinline__ bool kicking(const BatchCtrl& C, int forced, int nstages) {
   return C.active && forced && C.stage == nstages - 1;
 }
 
-// Per-instance fixed-order sums of tile partials.  mode 0: mean force; mode 1: scale.
+// Per-instance fixed-order sums of the group partials, one wave per instance slot: lane l
+// adds groups l, l + 64, ... in order, then a fixed butterfly.  mode 0: mean force;
+// mode 1: scale.
 template <typename T>
-__global__ void batch_kick_reduce_kernel(BatchTabs<T> b, int mode) {
-  const int n = b.meta[0];
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    BatchCtrl& C = b.ctrl[s];
-    const BatchInst<T>& I = b.inst[s];
-    if (!kicking(C, I.forced, I.nstages)) continue;   // per-thread loop over instances
-    double a0 = 0, a1 = 0, a2 = 0;
-    for (int k = I.tile0; k < I.tile0 + I.ntiles; ++k) {
-      a0 += b.partial[3 * (size_t)k + 0];
-      a1 += b.partial[3 * (size_t)k + 1];
-      a2 += b.partial[3 * (size_t)k + 2];
-    }
-    if (mode == 0) {
-      C.c.sum_rho = a0;
-      C.c.mu[0] = a1 / a0;
-      C.c.mu[1] = a2 / a0;
-    } else {
-      const double ncell = (double)I.nx * I.ny;
-      const double m0 = a0 / ncell, m1 = a1 / ncell;
-      const double dedt = C.c.power;
-      const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
-      double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
-      if (!(m0 > 1e-30)) scale = __builtin_nan("");
-      C.c.scale = scale;
-    }
+__global__ void __launch_bounds__(256) batch_kick_reduce_kernel(BatchTabs<T> b, int mode) {
+  const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  if (s >= b.meta[0]) return;
+  BatchCtrl& C = b.ctrl[s];
+  const BatchInst<T>& I = b.inst[s];
+  if (!kicking(C, I.forced, I.nstages)) return;
+  const int ng = I.nstrips * I.ngroups;
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int g = lane; g < ng; g += 64) {
+    a0 += I.part[3 * (size_t)g + 0];
+    a1 += I.part[3 * (size_t)g + 1];
+    a2 += I.part[3 * (size_t)g + 2];
+  }
+  wave_sum3(a0, a1, a2);
+  if (lane != 0) return;
+  if (mode == 0) {
+    C.c.sum_rho = a0;
+    C.c.mu[0] = a1 / a0;
+    C.c.mu[1] = a2 / a0;
+  } else {
+    const double ncell = (double)I.nx * I.ny;
+    const double m0 = a0 / ncell, m1 = a1 / ncell;
+    const double dedt = C.c.power;
+    const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
+    double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
+    if (!(m0 > 1e-30)) scale = __builtin_nan("");
+    C.c.scale = scale;
   }
 }
 
 // Second moments (mode 0) or the kick itself (mode 1) on one tile of a kicking instance.
 template <typename T>
-__device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl, int kt,
-                                int mode, double* partial) {
+__device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl, int mode) {
   const int t = threadIdx.x & 63;
   const T mu0 = T(C.c.mu[0]), mu1 = T(C.c.mu[1]);
   const T dt = T(C.c.dt);
@@ -547,18 +696,32 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
   const T scale = T(C.c.scale);
   const int i = tl.x0 + t;
   const bool on = t < kTileCols && i < I.nx;
-  const int yend = on ? min(tl.y0 + tl.rows, I.ny) : tl.y0;
-  double s0 = 0, s1 = 0;
+  const int yend = tl.y0 + tl.rows;
+  double s0 = 0, s1 = 0, s2 = 0;
   unsigned long long kmin = ~0ull;
+  // Rows are walked by every lane (uniform loop); lanes past the strip or the grid skip.
   for (int j = tl.y0; j < yend; ++j) {
     const size_t c = (size_t)j * I.nx + i;
     T* q = I.A + c * I.nv;
-    const T f0 = I.raw[c * 2] - mu0, f1 = I.raw[c * 2 + 1] - mu1;
     if (mode == 0) {
-      const T fsq = f0 * f0 + f1 * f1;
-      s0 += (double)(((T(0.5) * q[0]) * fsq) * dt);
-      s1 += (double)(q[1] * f0 + q[2] * f1);
-    } else {
+      if (on) {
+        const T f0 = I.raw[c * 2] - mu0, f1 = I.raw[c * 2 + 1] - mu1;
+        const T fsq = f0 * f0 + f1 * f1;
+        s0 += (double)(((T(0.5) * q[0]) * fsq) * dt);
+        s1 += (double)(q[1] * f0 + q[2] * f1);
+      }
+      if (((j & (kGroupRows - 1)) == kGroupRows - 1) || j == yend - 1) {
+        wave_sum3(s0, s1, s2);
+        if (t == 0) {
+          double* g = I.part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
+          g[0] = s0;
+          g[1] = s1;
+          g[2] = 0.0;
+        }
+        s0 = s1 = s2 = 0.0;
+      }
+    } else if (on) {
+      const T f0 = I.raw[c * 2] - mu0, f1 = I.raw[c * 2 + 1] - mu1;
       const T acc0 = off ? T(0) : scale * f0;
       const T acc1 = off ? T(0) : scale * f1;
       const T rho = q[0];
@@ -577,33 +740,26 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
       }
     }
   }
-  for (int off2 = 32; off2 > 0; off2 >>= 1) {
-    s0 += __shfl_xor(s0, off2);
-    s1 += __shfl_xor(s1, off2);
-    const unsigned long long o = __shfl_xor(kmin, off2);
-    kmin = o < kmin ? o : kmin;
-  }
-  if (t == 0) {
-    if (mode == 0) {
-      partial[3 * (size_t)kt + 0] = s0;
-      partial[3 * (size_t)kt + 1] = s1;
-      partial[3 * (size_t)kt + 2] = 0.0;
-    } else if (!I.smag_on) {
-      atomicMin(&C.c.dtkey, kmin);
+  if (mode == 1 && !I.smag_on) {
+    for (int off2 = 32; off2 > 0; off2 >>= 1) {
+      const unsigned long long o = __shfl_xor(kmin, off2);
+      kmin = o < kmin ? o : kmin;
     }
+    if (t == 0) atomicMin(&C.c.dtkey, kmin);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_kick_tiles_kernel(BatchTabs<T> b, int mode) {
-  const int ntiles = b.meta[1];
-  for (;;) {
-    const int kt = wave_next(&b.counters[mode == 0 ? CNT_MOM : CNT_APPLY]);
-    if (kt >= ntiles) break;
-    const Tile tl = b.tiles[kt];
-    BatchCtrl& C = b.ctrl[tl.inst];
-    const BatchInst<T>& I = b.inst[tl.inst];
-    if (kicking(C, I.forced, I.nstages)) batch_kick_tile<T>(I, C, tl, kt, mode, b.partial);
+  const int* L = b.lists + WL_KICK * list_stride(b.cap);
+  const int total = L[1];
+  unsigned* ctr = &b.counters[mode == 0 ? CNT_MOM : CNT_APPLY];
+  for (int k = wave_first(); k < total; k = wave_then(ctr)) {
+    int s, q;
+    list_item(b, L, k, s, q);
+    const BatchInst<T>& I = b.inst[s];
+    BatchCtrl& C = b.ctrl[s];
+    batch_kick_tile<T>(I, C, tile_of(I, C.rows, q), mode);
   }
 }
 
@@ -672,16 +828,18 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
 // that have just finished a step.
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_timestep_kernel(BatchTabs<T> b, int only) {
-  const int ntiles = b.meta[1];
-  for (;;) {
-    const int kt = wave_next(&b.counters[CNT_TSTEP]);
-    if (kt >= ntiles) break;
-    const Tile tl = b.tiles[kt];
-    BatchCtrl& C = b.ctrl[tl.inst];
-    const BatchInst<T>& I = b.inst[tl.inst];
-    const bool go = (only >= 0) ? (tl.inst == only)
-                                : (C.active && I.smag_on && C.stage == I.nstages - 1);
-    if (go) batch_timestep_tile<T>(I, C, tl);
+  const int* L = b.lists + WL_SMAG * list_stride(b.cap);
+  int total = L[1];
+  if (only >= 0) {
+    const BatchInst<T>& I = b.inst[only];
+    total = I.nstrips * ((I.ny + I.rows_cap - 1) / I.rows_cap);
+  }
+  for (int k = wave_first(); k < total; k = wave_then(&b.counters[CNT_TSTEP])) {
+    int s = only, q = k;
+    if (only < 0) list_item(b, L, k, s, q);
+    const BatchInst<T>& I = b.inst[s];
+    BatchCtrl& C = b.ctrl[s];
+    batch_timestep_tile<T>(I, C, tile_of(I, only >= 0 ? I.rows_cap : C.rows, q));
   }
 }
 

@@ -26,6 +26,7 @@ struct BatchBase {
   virtual int result(int id, void* U_out, double* time, double* dt, int* cycle) = 0;
   virtual int state_ptr(int id, void** p) = 0;
   virtual int set_graph(int on, int rounds) = 0;
+  virtual int set_fill(int fill) = 0;
   virtual long long pool_free() const = 0;
   virtual long long footprint(const jx_run_params* p, int forced) const = 0;
 };
@@ -38,8 +39,8 @@ inline bool batch_pow2(double x) {
 
 template <typename T> class Batch : public BatchBase {
  public:
-  Batch(int device, int max_inst, long long max_tiles, long long pool_bytes)
-      : device_(device), max_inst_(max_inst), max_tiles_(max_tiles), pool_bytes_(pool_bytes) {}
+  Batch(int device, int max_inst, long long pool_bytes)
+      : device_(device), max_inst_(max_inst), pool_bytes_(pool_bytes) {}
 
   int init() {
     if (hipSetDevice(device_) != hipSuccess) return fail("hipSetDevice failed");
@@ -52,12 +53,13 @@ template <typename T> class Batch : public BatchBase {
         per_cu < 1)
       per_cu = 1;
     grid_ = std::max(1, cus_) * per_cu;
+    nwaves_ = grid_ * (kBatchBlock / 64);
     if (hipMalloc(&d_inst_, sizeof(BatchInst<T>) * max_inst_) != hipSuccess ||
         hipMalloc(&d_ctrl_, sizeof(BatchCtrl) * max_inst_) != hipSuccess ||
-        hipMalloc(&d_tiles_, sizeof(Tile) * max_tiles_) != hipSuccess ||
         hipMalloc(&d_meta_, sizeof(int) * 8) != hipSuccess ||
         hipMalloc(&d_cnt_, sizeof(unsigned) * 8) != hipSuccess ||
-        hipMalloc(&d_partial_, sizeof(double) * 3 * max_tiles_) != hipSuccess ||
+        hipMalloc(&d_lists_, sizeof(int) * WL_TOTAL * list_stride(max_inst_)) != hipSuccess ||
+        hipMalloc(&d_flags_, sizeof(int) * max_inst_) != hipSuccess ||
         hipMalloc(&d_pool_, pool_bytes_) != hipSuccess ||
         hipHostMalloc(&h_ctrl_, sizeof(BatchCtrl) * max_inst_, hipHostMallocDefault) !=
             hipSuccess ||
@@ -66,6 +68,8 @@ template <typename T> class Batch : public BatchBase {
     hipMemsetAsync(d_ctrl_, 0, sizeof(BatchCtrl) * max_inst_, stream_);
     hipMemsetAsync(d_meta_, 0, sizeof(int) * 8, stream_);
     hipMemsetAsync(d_cnt_, 0, sizeof(unsigned) * 8, stream_);
+    hipMemsetAsync(d_lists_, 0, sizeof(int) * WL_TOTAL * list_stride(max_inst_), stream_);
+    hipMemsetAsync(d_flags_, 0, sizeof(int) * max_inst_, stream_);
     free_[0] = (size_t)pool_bytes_;
     h_inst_.resize(max_inst_);
     slots_.assign(max_inst_, Slot{});
@@ -77,8 +81,8 @@ template <typename T> class Batch : public BatchBase {
     if (stream_) hipStreamSynchronize(stream_);
     if (graph_exec_) hipGraphExecDestroy(graph_exec_);
     if (graph_) hipGraphDestroy(graph_);
-    for (void* p : {(void*)d_inst_, (void*)d_ctrl_, (void*)d_tiles_, (void*)d_meta_,
-                    (void*)d_cnt_, (void*)d_partial_, (void*)d_pool_})
+    for (void* p : {(void*)d_inst_, (void*)d_ctrl_, (void*)d_meta_, (void*)d_cnt_,
+                    (void*)d_pool_, (void*)d_lists_, (void*)d_flags_})
       if (p) hipFree(p);
     if (h_ctrl_) hipHostFree(h_ctrl_);
     if (h_meta_) hipHostFree(h_meta_);
@@ -88,7 +92,7 @@ template <typename T> class Batch : public BatchBase {
   long long footprint(const jx_run_params* p, int forced) const override {
     const size_t n = (size_t)p->nx * p->ny;
     size_t b = 4 * align(n * p->nvar * sizeof(T));
-    if (forced) b += align(n * 2 * sizeof(T));
+    if (forced) b += align(n * 2 * sizeof(T)) + align(group_bytes(p->nx, p->ny));
     return (long long)b;
   }
 
@@ -118,22 +122,15 @@ template <typename T> class Batch : public BatchBase {
     if (s < 0) return fail("add: no free instance slot");
     const size_t n = (size_t)p->nx * p->ny;
     const size_t sb = align(n * p->nvar * sizeof(T)), rb = f ? align(n * 2 * sizeof(T)) : 0;
-    const size_t need = 4 * sb + rb;
+    const size_t gb = f ? align(group_bytes(p->nx, p->ny)) : 0;
+    const size_t need = 4 * sb + rb + gb;
     long long off = alloc(need);
     if (off < 0) return fail("add: pool exhausted");
-    const int r = rows > 0 ? rows : 32;
-    const long long strips = (p->nx + kTileCols - 1) / kTileCols;
-    const long long segs = (p->ny + r - 1) / r;
-    if (ntiles_total() + strips * segs > max_tiles_) {
-      release((size_t)off, need);
-      return fail("add: tile list full");
-    }
     Slot& sl = slots_[s];
     sl = Slot{};
     sl.live = true;
     sl.off = (size_t)off;
     sl.bytes = need;
-    sl.rows = r;
     sl.p = *p;
     char* base = static_cast<char*>(d_pool_) + off;
     BatchInst<T> I{};
@@ -142,6 +139,11 @@ template <typename T> class Batch : public BatchBase {
     I.C = reinterpret_cast<T*>(base + 2 * sb);
     I.S = reinterpret_cast<T*>(base + 3 * sb);
     I.raw = f ? reinterpret_cast<T*>(base + 4 * sb) : nullptr;
+    I.part = f ? reinterpret_cast<double*>(base + 4 * sb + rb) : nullptr;
+    I.nstrips = (p->nx + kTileCols - 1) / kTileCols;
+    I.ngroups = (p->ny + kGroupRows - 1) / kGroupRows;
+    // Band heights are multiples of the 8-row group; 0 asks for the default cap.
+    I.rows_cap = std::max(kGroupRows, ((rows > 0 ? rows : 32) / kGroupRows) * kGroupRows);
     if (f) {
       I.coef = static_cast<const T*>(f->coefficients);
       I.cx = static_cast<const T*>(f->cos_x);
@@ -195,7 +197,7 @@ template <typename T> class Batch : public BatchBase {
         hipSuccess)
       return fail("add: state copy failed");
     if (I.raw) hipMemsetAsync(I.raw, 0, n * 2 * sizeof(T), stream_);
-    int rc = rebuild_tiles();
+    int rc = upload_tables();
     if (rc) return rc;
     BatchTabs<T> tb = tabs();
     hipLaunchKernelGGL(batch_init_kernel<T>, dim3(1), dim3(1), 0, stream_, tb, s);
@@ -211,7 +213,7 @@ template <typename T> class Batch : public BatchBase {
     hipLaunchKernelGGL(batch_retire_kernel<T>, dim3(1), dim3(1), 0, stream_, tabs(), id);
     release(slots_[id].off, slots_[id].bytes);
     slots_[id] = Slot{};
-    return rebuild_tiles();
+    return upload_tables();
   }
 
   int run(int rounds, int* active) override {
@@ -273,6 +275,15 @@ template <typename T> class Batch : public BatchBase {
     return JX_OK;
   }
 
+  // Tiles per resident wave that the per-round band heights aim for; 0 keeps every
+  // instance at its cap.  Kernel arguments change, so the graph is captured again.
+  int set_fill(int fill) override {
+    if (fill < 0 || fill > 64) return fail("fill must be in [0, 64]");
+    fill_ = fill;
+    drop_graph();
+    return JX_OK;
+  }
+
   int set_graph(int on, int rounds) override {
     if (rounds < 1 || rounds > 64) return fail("graph rounds must be in [1, 64]");
     use_graph_ = on != 0;
@@ -285,11 +296,15 @@ template <typename T> class Batch : public BatchBase {
   struct Slot {
     bool live = false;
     size_t off = 0, bytes = 0;
-    int rows = 32;
     jx_run_params p{};
   };
 
   static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
+  // Kick sums of a forced instance: 3 doubles per (58-column strip, 8-row group).
+  static size_t group_bytes(int nx, int ny) {
+    return (size_t)3 * sizeof(double) * ((nx + kTileCols - 1) / kTileCols) *
+           ((ny + kGroupRows - 1) / kGroupRows);
+  }
   static constexpr size_t smem() { return (size_t)4 * (3 * kBatchNV + 9) * 64 * sizeof(T); }
 
   int fail(const std::string& m, int code = JX_EINVAL) {
@@ -298,7 +313,8 @@ template <typename T> class Batch : public BatchBase {
   }
 
   BatchTabs<T> tabs() const {
-    return BatchTabs<T>{d_inst_, d_ctrl_, d_tiles_, d_meta_, d_cnt_, d_partial_};
+    return BatchTabs<T>{d_inst_, d_ctrl_, d_meta_, d_cnt_, d_lists_, d_flags_,
+                        max_inst_, nwaves_, fill_};
   }
 
   long long alloc(size_t need) {
@@ -331,48 +347,15 @@ template <typename T> class Batch : public BatchBase {
     }
   }
 
-  long long ntiles_total() const {
-    long long t = 0;
-    for (int k = 0; k < max_inst_; ++k)
-      if (slots_[k].live) {
-        const jx_run_params& p = slots_[k].p;
-        t += (long long)((p.nx + kTileCols - 1) / kTileCols) *
-             ((p.ny + slots_[k].rows - 1) / slots_[k].rows);
-      }
-    return t;
-  }
-
-  // Tiles of every live instance, largest instances first; uploads tiles, instance table
-  // and the meta counts (the graph reads them from device memory, so it stays valid).
-  int rebuild_tiles() {
-    std::vector<int> order;
+  // Uploads the instance table and the number of slots in use (the graph reads both from
+  // device memory, so it stays valid across adds and removes).
+  int upload_tables() {
     int top = 0;
     for (int k = 0; k < max_inst_; ++k)
-      if (slots_[k].live) {
-        order.push_back(k);
-        top = k + 1;
-      }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      return (long long)slots_[a].p.nx * slots_[a].p.ny > (long long)slots_[b].p.nx * slots_[b].p.ny;
-    });
-    h_tiles_.clear();
-    for (int k : order) {
-      const jx_run_params& p = slots_[k].p;
-      const int r = slots_[k].rows;
-      h_inst_[k].tile0 = (int)h_tiles_.size();
-      for (int y = 0; y < p.ny; y += r)
-        for (int x = 0; x < p.nx; x += kTileCols) h_tiles_.push_back(Tile{k, x, y, r});
-      h_inst_[k].ntiles = (int)h_tiles_.size() - h_inst_[k].tile0;
-    }
-    if ((long long)h_tiles_.size() > max_tiles_) return fail("tile list full");
-    int meta[2] = {top, (int)h_tiles_.size()};
-    if (!h_tiles_.empty() &&
-        hipMemcpyAsync(d_tiles_, h_tiles_.data(), sizeof(Tile) * h_tiles_.size(),
-                       hipMemcpyHostToDevice, stream_) != hipSuccess)
-      return fail("tile upload failed");
+      if (slots_[k].live) top = k + 1;
     if (hipMemcpyAsync(d_inst_, h_inst_.data(), sizeof(BatchInst<T>) * max_inst_,
                        hipMemcpyHostToDevice, stream_) != hipSuccess ||
-        hipMemcpyAsync(d_meta_, meta, sizeof(meta), hipMemcpyHostToDevice, stream_) !=
+        hipMemcpyAsync(d_meta_, &top, sizeof(int), hipMemcpyHostToDevice, stream_) !=
             hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
       return fail("table upload failed");
@@ -382,16 +365,14 @@ template <typename T> class Batch : public BatchBase {
   void launch_round() {
     const BatchTabs<T> tb = tabs();
     const dim3 g(grid_), blk(kBatchBlock);
-    const dim3 gi((max_inst_ + 255) / 256);
-    hipLaunchKernelGGL(batch_control_kernel<T>, dim3(1), dim3(256), 0, stream_, tb);
-    hipLaunchKernelGGL(batch_render_kernel<T>, g, blk, 0, stream_, tb);
+    const dim3 gi((max_inst_ + 3) / 4);   // one wave per instance slot
+    hipLaunchKernelGGL(batch_control_kernel<T>, dim3(1), dim3(kCtrlBlock), 0, stream_, tb);
     hipLaunchKernelGGL(batch_stage_kernel<T>, g, blk, smem(), stream_, tb);
     hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 0);
     hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 0);
     hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 1);
     hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 1);
     hipLaunchKernelGGL(batch_timestep_kernel<T>, g, blk, 0, stream_, tb, -1);
-    hipLaunchKernelGGL(batch_advance_kernel<T>, dim3(1), dim3(256), 0, stream_, tb);
   }
 
   int capture() {
@@ -415,21 +396,21 @@ template <typename T> class Batch : public BatchBase {
 
   int device_;
   int max_inst_;
-  long long max_tiles_, pool_bytes_;
+  long long pool_bytes_;
   hipStream_t stream_ = nullptr;
-  int cus_ = 0, grid_ = 1;
+  int cus_ = 0, grid_ = 1, nwaves_ = 4;
+  int fill_ = 2;   // tiles per resident wave that the band heights aim for
   BatchInst<T>* d_inst_ = nullptr;
   BatchCtrl* d_ctrl_ = nullptr;
-  Tile* d_tiles_ = nullptr;
   int* d_meta_ = nullptr;
   unsigned* d_cnt_ = nullptr;
-  double* d_partial_ = nullptr;
+  int* d_lists_ = nullptr;
+  int* d_flags_ = nullptr;
   void* d_pool_ = nullptr;
   BatchCtrl* h_ctrl_ = nullptr;
   int* h_meta_ = nullptr;
   std::vector<BatchInst<T>> h_inst_;
   std::vector<Slot> slots_;
-  std::vector<Tile> h_tiles_;
   std::map<size_t, size_t> free_;
   bool use_graph_ = true;
   int graph_rounds_ = 6;

@@ -14,9 +14,9 @@
 #include "jx_batch_host.hpp"
 #include "jx_stage.hpp"
 
-jx::BatchBase* jx_make_batch_f32(int device, int max_inst, long long max_tiles,
+jx::BatchBase* jx_make_batch_f32(int device, int max_inst,
                                  long long pool_bytes, std::string* err);
-jx::BatchBase* jx_make_batch_f64(int device, int max_inst, long long max_tiles,
+jx::BatchBase* jx_make_batch_f64(int device, int max_inst,
                                  long long pool_bytes, std::string* err);
 
 struct jx_batch {
@@ -829,16 +829,16 @@ int jx_conserved_sums(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void
 
 // ---- batched ensemble engine ----------------------------------------------------------
 
-int jx_batch_create(int device, int dtype, int max_instances, long long max_tiles,
-                    long long pool_bytes, jx_batch** out) {
+int jx_batch_create(int device, int dtype, int max_instances, long long pool_bytes,
+                    jx_batch** out) {
   if (!out) return JX_EINVAL;
   *out = nullptr;
-  if ((dtype != 0 && dtype != 1) || max_instances < 1 || max_tiles < 1 || pool_bytes < 256)
+  if ((dtype != 0 && dtype != 1) || max_instances < 1 || pool_bytes < 256)
     return JX_EINVAL;
   std::string err;
   jx::BatchBase* impl = dtype == 1
-      ? jx_make_batch_f32(device, max_instances, max_tiles, pool_bytes, &err)
-      : jx_make_batch_f64(device, max_instances, max_tiles, pool_bytes, &err);
+      ? jx_make_batch_f32(device, max_instances, pool_bytes, &err)
+      : jx_make_batch_f64(device, max_instances, pool_bytes, &err);
   if (!impl) {
     std::fprintf(stderr, "jx_batch_create: %s\n", err.c_str());
     return JX_EHIP;
@@ -886,6 +886,10 @@ int jx_batch_state(jx_batch* b, int id, void** state_out) {
 
 int jx_batch_set_graph(jx_batch* b, int use_graph, int rounds_per_graph) {
   return b ? b->impl->set_graph(use_graph, rounds_per_graph) : JX_EINVAL;
+}
+
+int jx_batch_set_fill(jx_batch* b, int fill) {
+  return b ? b->impl->set_fill(fill) : JX_EINVAL;
 }
 
 }  // extern "C"

@@ -20,8 +20,8 @@ from .hydro import run_params
 
 
 class Ensemble:
-    def __init__(self, dtype=torch.float32, max_instances=512, max_tiles=1 << 20,
-                 pool_bytes=None, device=0, graph=True, graph_rounds=6):
+    def __init__(self, dtype=torch.float32, max_instances=512, pool_bytes=None, device=0,
+                 graph=True, graph_rounds=6):
         if not torch.cuda.is_available():
             raise RuntimeError("Ensemble needs a ROCm GPU (gfx950); none is visible")
         self.dtype = dtype
@@ -33,8 +33,8 @@ class Ensemble:
         self.pool_bytes = int(pool_bytes)
         h = ctypes.c_void_p()
         lib = _lib.load()
-        rc = lib.jx_batch_create(int(device), did, int(max_instances), int(max_tiles),
-                                 self.pool_bytes, ctypes.byref(h))
+        rc = lib.jx_batch_create(int(device), did, int(max_instances), self.pool_bytes,
+                                 ctypes.byref(h))
         if rc != 0:
             raise _lib.JxError("jx_batch_create failed (see stderr)")
         self.handle = h
@@ -63,6 +63,11 @@ class Ensemble:
     def set_graph(self, on=True, rounds=6):
         self._check(_lib.load().jx_batch_set_graph(self.handle, int(bool(on)), int(rounds)),
                     "set_graph")
+
+    def set_fill(self, fill=2):
+        """Tiles per resident wave that each round's tile heights aim for; 0 keeps every
+        instance at its own cap.  Results do not depend on it."""
+        self._check(_lib.load().jx_batch_set_fill(self.handle, int(fill)), "set_fill")
 
     def params(self, U0, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, **kw):
         U0 = as_device(U0, dtype=self.dtype, device=self.device)

@@ -140,3 +140,26 @@ def test_remove_and_reuse_slots(jx):
     np.testing.assert_array_equal(_np(got[0]), _np(ref[0]))
     with pytest.raises(KeyError):
         ens.add(U, 1 / 64, 1 / 64, 1.0, 1e-10, "rk3", 0.4, 0.02, 100, reconstruct="weno9")
+
+
+def test_results_do_not_depend_on_tile_height(jx):
+    """Forced and Smagorinsky instances (kick sums, CFL and viscous limits) give bitwise the
+    same result whatever band height the rounds cut them into."""
+    from jathenax_amd.ensemble import Ensemble
+    specs = [_specs()[-1], _specs()[1]]
+
+    def run(rows, fill):
+        ens = Ensemble(dtype=torch.float32, max_instances=4, pool_bytes=1 << 26)
+        ens.set_fill(fill)
+        ids = [ens.add(torch.tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"],
+                       s["dy"], 1.0, 1e-10, s["integ"], 0.4, 0.03, 10 ** 6, rows=rows, **s["kw"])
+               for s in specs]
+        ens.run_until_done(chunk=12)
+        return [ens.result(i) for i in ids]
+
+    ref = run(8, 2)
+    for rows, fill in ((64, 0), (24, 0), (8, 0), (64, 2)):
+        got = run(rows, fill)
+        for a, b in zip(ref, got):
+            np.testing.assert_array_equal(_np(a[0]), _np(b[0]))
+            assert a[1:] == b[1:]
