@@ -17,8 +17,8 @@ constexpr int kAuxBlock = 256;
 // W (isothermal_timestep's argument); otherwise it holds U and is recovered on read.
 // ---------------------------------------------------------------------------------------
 
-template <typename T>
-__device__ __forceinline__ void load_prim(const T* q, int nv, bool primitive, T dfloor, T& r,
+template <typename T, typename Ptr>   // Ptr: const T* in any address space
+__device__ __forceinline__ void load_prim(Ptr q, int nv, bool primitive, T dfloor, T& r,
                                           T& vx, T& vy) {
   (void)nv;
   if (primitive) {

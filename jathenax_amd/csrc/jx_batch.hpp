@@ -43,6 +43,7 @@ template <typename T> struct BatchInst {
   const T* coef; const T* cx; const T* sx; const T* cy; const T* sy;
   int nx, ny, nv, recon, rs, nstages;
   int visc_on, smag_on, drag_on, forced, n_updates, n_modes;
+  long long reg_stride;  // elements from A to B, B to C, C to S
   int nstrips, ngroups;  // 58-column strips, 8-row groups
   int rows_cap;          // largest band height (multiple of 8)
   int pad_;
@@ -87,13 +88,41 @@ template <typename T> struct BatchTabs {
   int fill;               // tiles per wave the band heights aim for
 };
 
+// Pointers read from the instance table are generic (flat) to the compiler.  Flat accesses
+// also count against the LDS wait counter, so every LDS wait in the row march would drain
+// the row prefetch; casting them to the global address space keeps them on the vector
+// memory path.
+template <typename T> using gmem_t = T __attribute__((address_space(1)));
+template <typename T> __device__ __forceinline__ gmem_t<T>* gmem(T* p) {
+  return (gmem_t<T>*)p;
+}
+template <typename T> __device__ __forceinline__ const gmem_t<T>* gmem(const T* p) {
+  return (const gmem_t<T>*)p;
+}
+
+// The instance table (written by the host between rounds) and the work lists (written by
+// the control kernel, an earlier launch) are read-only while a worker kernel runs, so they
+// are read through the constant address space: uniform addresses then become scalar loads
+// into SGPRs instead of vector loads into VGPRs.
+template <typename T> using cmem_t = T __attribute__((address_space(4)));
+template <typename S> __device__ __forceinline__ S load_const(const S* p) {
+  static_assert(sizeof(S) % 4 == 0, "record must be a whole number of dwords");
+  S out;
+  const cmem_t<int>* src = (const cmem_t<int>*)p;
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(S) / 4); ++i) dst[i] = src[i];
+  return out;
+}
+
 // k-th tile of work list L (k < total): instance slot s and its tile q.
 template <typename T>
 __device__ __forceinline__ void list_item(const BatchTabs<T>& b, const int* L, int k, int& s,
                                           int& q) {
-  const int n = L[0];
-  const int* pref = L + 2;
-  const int* ids = pref + (b.cap + 1);
+  const cmem_t<int>* Lc = (const cmem_t<int>*)L;
+  const int n = Lc[0];
+  const cmem_t<int>* pref = Lc + 2;
+  const cmem_t<int>* ids = pref + (b.cap + 1);
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -150,43 +179,26 @@ __device__ __forceinline__ int wave_then(unsigned* counter) {
   return (int)(gridDim.x * (blockDim.x >> 6)) + wave_next(counter);
 }
 
-// RK rows (hydro.py:20-36) by number of stages and stage index.
+// RK rows (gam0, gam1, beta, delta) of RK2, RK3 and low-storage RK4 by stage
+// (hydro.py:20-36), in constant memory so that a runtime stage index reads them with
+// scalar loads.
+static __constant__ double kRKRows[3][4][4] = {
+    {{0.0, 1.0, 1.0, 0.0}, {0.5, 0.5, 0.5, 0.0}, {0, 0, 0, 0}, {0, 0, 0, 0}},
+    {{0.0, 1.0, 1.0, 0.0}, {0.25, 0.75, 0.25, 0.0}, {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0.0},
+     {0, 0, 0, 0}},
+    {{0.0, 1.0, 1.193743905974738, 0.0},
+     {0.121098479554482, 0.721781678111411, 0.099279895495783, 0.217683334308543},
+     {-3.843833699660025, 2.121209265338722, 1.131678018054042, 1.065841341361089},
+     {0.546370891121863, 0.198653035682705, 0.310665766509336, 0.0}}};
+
 __device__ __forceinline__ void rk_row(int ns, int k, double& g0, double& g1, double& beta,
                                        double& delta, int& has_delta) {
-  has_delta = 0;
-  delta = 0.0;
-  if (ns == 2) {
-    g0 = k == 0 ? 0.0 : 0.5;
-    g1 = k == 0 ? 1.0 : 0.5;
-    beta = k == 0 ? 1.0 : 0.5;
-  } else if (ns == 3) {
-    g0 = k == 0 ? 0.0 : k == 1 ? 0.25 : 2.0 / 3.0;
-    g1 = k == 0 ? 1.0 : k == 1 ? 0.75 : 1.0 / 3.0;
-    beta = k == 0 ? 1.0 : k == 1 ? 0.25 : 2.0 / 3.0;
-  } else {
-    const double G0[4] = {0.0, 0.121098479554482, -3.843833699660025, 0.546370891121863};
-    const double G1[4] = {1.0, 0.721781678111411, 2.121209265338722, 0.198653035682705};
-    const double BE[4] = {1.193743905974738, 0.099279895495783, 1.131678018054042,
-                          0.310665766509336};
-    const double DE[4] = {0.0, 0.217683334308543, 1.065841341361089, 0.0};
-    g0 = G0[k];
-    g1 = G1[k];
-    beta = BE[k];
-    delta = DE[k];
-    has_delta = k > 0;
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void edges_rt(int recon, T qm2, T qm1, T q, T qp1, T qp2, T eps,
-                                         T& lo, T& hi) {
-  switch (recon) {
-    case RECON_DC: cell_edges<RECON_DC>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
-    case RECON_PLM: cell_edges<RECON_PLM>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
-    case RECON_PPM4: cell_edges<RECON_PPM4>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
-    case RECON_PPMX: cell_edges<RECON_PPMX>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
-    default: cell_edges<RECON_WENOZ>(qm2, qm1, q, qp1, qp2, eps, lo, hi); break;
-  }
+  const double* row = kRKRows[ns - 2][k];
+  g0 = row[0];
+  g1 = row[1];
+  beta = row[2];
+  delta = row[3];
+  has_delta = ns == 4 && k > 0;
 }
 
 template <typename T>
@@ -205,9 +217,12 @@ __device__ __forceinline__ T over_rt(T x, T d, T inv) {
   else return x / d;
 }
 
-// One RK stage of one instance (NV variables) over one tile, executed by one wave.  Same
-// arithmetic as stage_kernel; `lds` is this wave's slice (3 * kBatchNV + 9 rows of 64).
-template <typename T, int NV>
+// One RK stage of one instance (NV variables, reconstruction RECON) over one tile, executed
+// by one wave.  Same arithmetic as stage_kernel; `lds` is this wave's slice
+// (3 * kBatchNV + 9 rows of 64).  The Riemann solver and the physics switches stay runtime
+// choices (uniform branches); the reconstruction is compiled in, so that the LDS reads and
+// the stencil arithmetic of all variables can be scheduled together.
+template <typename T, int NV, int RECON>
 __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtrl& C,
                                                  const Tile& tl, T* lds) {
   constexpr int R = 5;
@@ -236,23 +251,30 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
   rk_row(ns, k, g0, g1, beta, delta, has_delta);
   const bool blended = (g0 != 0.0 && g0 + g1 == 1.0);
   const bool last = (k == ns - 1);
-  const T* u_in = I.A;
-  T* u_out = I.B;
-  const T* u_sv = nullptr;
-  T* s_out = nullptr;
-  bool floor_saved = false;
+  // Registers A, B, C, S sit reg_stride elements apart, so the plan is kept as register
+  // numbers (integer selects) rather than as a choice between pointers.
+  int in_r = 0, out_r = 1, sv_r = 0, so_r = 0;
+  bool floor_saved = false, has_sv = false;
   if (k > 0) {
-    if (ns == 2) { u_in = I.B; u_out = I.A; }
-    else if (ns == 3) { u_in = (k == 1) ? I.B : I.C; u_out = (k == 1) ? I.C : I.A; }
+    if (ns == 2) { in_r = 1; out_r = 0; }
+    else if (ns == 3) { in_r = k; out_r = (k == 1) ? 2 : 0; }
     else {
-      u_in = (k == 1) ? I.B : (k == 2) ? I.C : I.B;
-      u_out = (k == 1) ? I.C : (k == 2) ? I.B : I.A;
-      s_out = I.S;
+      in_r = (k == 2) ? 2 : 1;
+      out_r = (k == 1) ? 2 : (k == 2) ? 1 : 0;
+      so_r = 3;
     }
     const bool from_a = (ns != 4) || (k == 1);
-    u_sv = from_a ? I.A : I.S;
+    sv_r = from_a ? 0 : 3;
     floor_saved = from_a;
+    has_sv = true;
   }
+  gmem_t<T>* const regs = gmem(I.A);
+  const gmem_t<T>* g_in = regs + (size_t)in_r * I.reg_stride;
+  gmem_t<T>* g_out = regs + (size_t)out_r * I.reg_stride;
+  const gmem_t<T>* g_sv = regs + (size_t)sv_r * I.reg_stride;
+  gmem_t<T>* g_sout = regs + (size_t)so_r * I.reg_stride;
+  const gmem_t<T>* g_raw = gmem(static_cast<const T*>(I.raw));
+  gmem_t<double>* g_part = gmem(I.part);
   const bool kick = last && I.forced;
   const bool want_key = last && !I.forced && !I.smag_on;
 
@@ -263,7 +285,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
   const T dfloor = I.dfloor;
   const T drag_coef = sdt * I.drag;
   const bool visc_on = I.visc_on != 0;
-  const bool floors = I.recon == RECON_PPMX || I.recon == RECON_WENOZ;
+  constexpr bool floors = RECON == RECON_PPMX || RECON == RECON_WENOZ;
 
   T w[R][NV], u[R][NV], hy[NV], gp[NV], gn[NV], pf[NV];
 #pragma unroll
@@ -275,7 +297,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
   {
     int yn = (y0 - 3) % ny;
     if (yn < 0) yn += ny;
-    const T* src = u_in + ((size_t)yn * nx + xc) * nv;
+    const gmem_t<T>* src = g_in + ((size_t)yn * nx + xc) * nv;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
       pf[v] = src[v];
@@ -293,7 +315,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     if (i + 1 < nit) {
       int yn = (r + 1) % ny;
       if (yn < 0) yn += ny;
-      const T* src = u_in + ((size_t)yn * nx + xc) * nv;
+      const gmem_t<T>* src = g_in + ((size_t)yn * nx + xc) * nv;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         pf[v] = src[v];
@@ -301,8 +323,8 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     T svr[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) svr[v] = T(0);
-    if (i >= 6 && u_sv) {
-      const T* sv = u_sv + ((size_t)(r - 3) * nx + xc) * nv;
+    if (i >= 6 && has_sv) {
+      const gmem_t<T>* sv = g_sv + ((size_t)(r - 3) * nx + xc) * nv;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         svr[v] = sv[v];
@@ -331,7 +353,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
       T lo[NV], hi[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        edges_rt<T>(I.recon, w[S4][v], w[S3][v], w[S2][v], w[S1][v], w[S0][v], I.eps, lo[v],
+        cell_edges<RECON>(w[S4][v], w[S3][v], w[S2][v], w[S1][v], w[S0][v], I.eps, lo[v],
                     hi[v]);
       if (floors) {
         lo[0] = fmx(lo[0], dfloor);
@@ -368,7 +390,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const T* row = sW + v * 64;
-        edges_rt<T>(I.recon, row[tm2], row[tm1], row[t], row[tp1], row[tp2], I.eps, xl[v], xh[v]);
+        cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], I.eps, xl[v], xh[v]);
       }
       if (floors) {
         xl[0] = fmx(xl[0], dfloor);
@@ -418,7 +440,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
             us[v] = us[v] + T(delta) * u[S3][v];
-            s_out[cell * nv + v] = us[v];
+            g_sout[cell * nv + v] = us[v];
           }
         }
 #pragma unroll
@@ -436,7 +458,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
           un[2] = un[2] + (-(c * w[S3][2]));
         }
         if (kick) {
-          const T* rf = I.raw + cell * 2;
+          const gmem_t<T>* rf = g_raw + cell * 2;
           acc0 += (double)un[0];
           acc1 += (double)(un[0] * rf[0]);
           acc2 += (double)(un[0] * rf[1]);
@@ -453,14 +475,14 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-          u_out[cell * nv + v] = un[v];
+          g_out[cell * nv + v] = un[v];
       }
       // Kick sums close at the end of every 8-row group (bands start on group bounds).
       const int j = r - 3;
       if (kick && (((j & (kGroupRows - 1)) == kGroupRows - 1) || j == yend - 1)) {
         wave_sum3(acc0, acc1, acc2);
         if (t == 0) {
-          double* g = I.part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
+          gmem_t<double>* g = g_part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
           g[0] = acc0;
           g[1] = acc1;
           g[2] = acc2;
@@ -494,20 +516,34 @@ template <typename T>
 __device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C,
                                                   const Tile& tl);
 
+template <typename T, int NV>
+__device__ __forceinline__ void batch_stage_dispatch(const BatchInst<T>& I, BatchCtrl& C,
+                                                     const Tile& tl, T* lds) {
+  switch (I.recon) {
+    case RECON_DC: batch_stage_tile<T, NV, RECON_DC>(I, C, tl, lds); break;
+    case RECON_PLM: batch_stage_tile<T, NV, RECON_PLM>(I, C, tl, lds); break;
+    case RECON_PPM4: batch_stage_tile<T, NV, RECON_PPM4>(I, C, tl, lds); break;
+    case RECON_PPMX: batch_stage_tile<T, NV, RECON_PPMX>(I, C, tl, lds); break;
+    default: batch_stage_tile<T, NV, RECON_WENOZ>(I, C, tl, lds); break;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_stage_kernel(BatchTabs<T> b) {
   extern __shared__ double bsm[];
   T* lds = reinterpret_cast<T*>(bsm) + (threadIdx.x >> 6) * (3 * kBatchNV + 9) * 64;
   const int* L = b.lists + WL_STAGE * list_stride(b.cap);
-  const int total = L[1];
+  const int total = ((const cmem_t<int>*)L)[1];
   for (int k = wave_first(); k < total; k = wave_then(&b.counters[CNT_STAGE])) {
     int s, q;
     list_item(b, L, k, s, q);
-    const BatchInst<T>& I = b.inst[s];
+    // By-value copy: the tile's stores to the state could alias the instance record, so
+    // a reference would make the compiler reload its fields on every row.
+    const BatchInst<T> I = load_const(b.inst + s);
     BatchCtrl& C = b.ctrl[s];
     const Tile tl = tile_of(I, C.rows, q);
-    if (I.nv == 4) batch_stage_tile<T, 4>(I, C, tl, lds);
-    else batch_stage_tile<T, 3>(I, C, tl, lds);
+    if (I.nv == 4) batch_stage_dispatch<T, 4>(I, C, tl, lds);
+    else batch_stage_dispatch<T, 3>(I, C, tl, lds);
     // A step that opened a new forcing row renders the raw force of this tile (read by
     // the kick at the step's final stage).
     if (C.stage == 0 && I.forced && C.c.render) batch_render_tile<T>(I, C, tl);
@@ -624,7 +660,9 @@ __device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const B
   const int t = threadIdx.x & 63;
   const int row = C.c.update;
   const bool fill = (row < 0) || (row >= I.n_updates);
-  const T* amp = I.coef + (size_t)(fill ? 0 : row) * I.n_modes * 4;
+  const gmem_t<T>* amp = gmem(I.coef) + (size_t)(fill ? 0 : row) * I.n_modes * 4;
+  const gmem_t<T>*cx = gmem(I.cx), *sx = gmem(I.sx), *cy = gmem(I.cy), *sy = gmem(I.sy);
+  gmem_t<T>* raw = gmem(I.raw);
   const T nan = T(__builtin_nan(""));
   const int i = tl.x0 + t;
   const bool on = t < kTileCols && i < I.nx;   // per lane; the tile loop stays uniform
@@ -632,18 +670,18 @@ __device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const B
   for (int j = tl.y0; j < yend; ++j) {
     T f0 = T(0), f1 = T(0);
     for (int m = 0; m < I.n_modes; ++m) {
-      const T cxv = I.cx[(size_t)m * I.nx + i], sxv = I.sx[(size_t)m * I.nx + i];
-      const T cyv = I.cy[(size_t)m * I.ny + j], syv = I.sy[(size_t)m * I.ny + j];
+      const T cxv = cx[(size_t)m * I.nx + i], sxv = sx[(size_t)m * I.nx + i];
+      const T cyv = cy[(size_t)m * I.ny + j], syv = sy[(size_t)m * I.ny + j];
       const T re = cxv * cyv - sxv * syv;
       const T im = cxv * syv + sxv * cyv;
-      const T* am = amp + (size_t)m * 4;
+      const gmem_t<T>* am = amp + (size_t)m * 4;
       const T a00 = fill ? nan : am[0], a01 = fill ? nan : am[1];
       const T a10 = fill ? nan : am[2], a11 = fill ? nan : am[3];
       f0 = f0 + (re * a00 - im * a01);
       f1 = f1 + (re * a10 - im * a11);
     }
-    I.raw[((size_t)j * I.nx + i) * 2 + 0] = f0;
-    I.raw[((size_t)j * I.nx + i) * 2 + 1] = f1;
+    raw[((size_t)j * I.nx + i) * 2 + 0] = f0;
+    raw[((size_t)j * I.nx + i) * 2 + 1] = f1;
   }
 }
 
@@ -663,11 +701,12 @@ __global__ void __launch_bounds__(256) batch_kick_reduce_kernel(BatchTabs<T> b, 
   const BatchInst<T>& I = b.inst[s];
   if (!kicking(C, I.forced, I.nstages)) return;
   const int ng = I.nstrips * I.ngroups;
+  const gmem_t<double>* part = gmem(static_cast<const double*>(I.part));
   double a0 = 0, a1 = 0, a2 = 0;
   for (int g = lane; g < ng; g += 64) {
-    a0 += I.part[3 * (size_t)g + 0];
-    a1 += I.part[3 * (size_t)g + 1];
-    a2 += I.part[3 * (size_t)g + 2];
+    a0 += part[3 * (size_t)g + 0];
+    a1 += part[3 * (size_t)g + 1];
+    a2 += part[3 * (size_t)g + 2];
   }
   wave_sum3(a0, a1, a2);
   if (lane != 0) return;
@@ -699,13 +738,16 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
   const int yend = tl.y0 + tl.rows;
   double s0 = 0, s1 = 0, s2 = 0;
   unsigned long long kmin = ~0ull;
+  gmem_t<T>* A = gmem(I.A);
+  const gmem_t<T>* raw = gmem(static_cast<const T*>(I.raw));
+  gmem_t<double>* part = gmem(I.part);
   // Rows are walked by every lane (uniform loop); lanes past the strip or the grid skip.
   for (int j = tl.y0; j < yend; ++j) {
     const size_t c = (size_t)j * I.nx + i;
-    T* q = I.A + c * I.nv;
+    gmem_t<T>* q = A + c * I.nv;
     if (mode == 0) {
       if (on) {
-        const T f0 = I.raw[c * 2] - mu0, f1 = I.raw[c * 2 + 1] - mu1;
+        const T f0 = raw[c * 2] - mu0, f1 = raw[c * 2 + 1] - mu1;
         const T fsq = f0 * f0 + f1 * f1;
         s0 += (double)(((T(0.5) * q[0]) * fsq) * dt);
         s1 += (double)(q[1] * f0 + q[2] * f1);
@@ -713,7 +755,7 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
       if (((j & (kGroupRows - 1)) == kGroupRows - 1) || j == yend - 1) {
         wave_sum3(s0, s1, s2);
         if (t == 0) {
-          double* g = I.part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
+          gmem_t<double>* g = part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
           g[0] = s0;
           g[1] = s1;
           g[2] = 0.0;
@@ -721,7 +763,7 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
         s0 = s1 = s2 = 0.0;
       }
     } else if (on) {
-      const T f0 = I.raw[c * 2] - mu0, f1 = I.raw[c * 2 + 1] - mu1;
+      const T f0 = raw[c * 2] - mu0, f1 = raw[c * 2 + 1] - mu1;
       const T acc0 = off ? T(0) : scale * f0;
       const T acc1 = off ? T(0) : scale * f1;
       const T rho = q[0];
@@ -752,12 +794,12 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_kick_tiles_kernel(BatchTabs<T> b, int mode) {
   const int* L = b.lists + WL_KICK * list_stride(b.cap);
-  const int total = L[1];
+  const int total = ((const cmem_t<int>*)L)[1];
   unsigned* ctr = &b.counters[mode == 0 ? CNT_MOM : CNT_APPLY];
   for (int k = wave_first(); k < total; k = wave_then(ctr)) {
     int s, q;
     list_item(b, L, k, s, q);
-    const BatchInst<T>& I = b.inst[s];
+    const BatchInst<T> I = load_const(b.inst + s);   // by value (see batch_stage_kernel)
     BatchCtrl& C = b.ctrl[s];
     batch_kick_tile<T>(I, C, tile_of(I, C.rows, q), mode);
   }
@@ -771,10 +813,11 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
   const bool on = t < kTileCols && i < I.nx;
   const int nx = I.nx, ny = I.ny, nv = I.nv;
   unsigned long long kmin = ~0ull, kmax = 0ull;
+  const gmem_t<T>* A = gmem(static_cast<const T*>(I.A));
   const int yend = on ? min(tl.y0 + tl.rows, ny) : tl.y0;
   for (int j = tl.y0; j < yend; ++j) {
     T r, vx, vy;
-    load_prim(I.A + ((size_t)j * nx + i) * nv, nv, false, I.dfloor, r, vx, vy);
+    load_prim(A + ((size_t)j * nx + i) * nv, nv, false, I.dfloor, r, vx, vy);
     const unsigned long long kx = key_of((double)dvd(I.dx, fabs(vx) + I.cs));
     const unsigned long long ky = key_of((double)dvd(I.dy, fabs(vy) + I.cs));
     kmin = kx < kmin ? kx : kmin;
@@ -784,7 +827,7 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
       const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
       auto at = [&](int jj, int ii, T& u_, T& v_) {
         T rr;
-        load_prim(I.A + ((size_t)jj * nx + ii) * nv, nv, false, I.dfloor, rr, u_, v_);
+        load_prim(A + ((size_t)jj * nx + ii) * nv, nv, false, I.dfloor, rr, u_, v_);
       };
       T u_w, v_w, u_n, v_n, u_nw, v_nw, u_s, v_s, u_sw, v_sw, u_e, v_e, u_se, v_se;
       at(j, im, u_w, v_w);
@@ -829,7 +872,7 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_timestep_kernel(BatchTabs<T> b, int only) {
   const int* L = b.lists + WL_SMAG * list_stride(b.cap);
-  int total = L[1];
+  int total = ((const cmem_t<int>*)L)[1];
   if (only >= 0) {
     const BatchInst<T>& I = b.inst[only];
     total = I.nstrips * ((I.ny + I.rows_cap - 1) / I.rows_cap);
@@ -837,7 +880,7 @@ __global__ void __launch_bounds__(kBatchBlock) batch_timestep_kernel(BatchTabs<T
   for (int k = wave_first(); k < total; k = wave_then(&b.counters[CNT_TSTEP])) {
     int s = only, q = k;
     if (only < 0) list_item(b, L, k, s, q);
-    const BatchInst<T>& I = b.inst[s];
+    const BatchInst<T> I = load_const(b.inst + s);   // by value (see batch_stage_kernel)
     BatchCtrl& C = b.ctrl[s];
     batch_timestep_tile<T>(I, C, tile_of(I, only >= 0 ? I.rows_cap : C.rows, q));
   }

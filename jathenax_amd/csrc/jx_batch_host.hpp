@@ -140,6 +140,7 @@ template <typename T> class Batch : public BatchBase {
     I.S = reinterpret_cast<T*>(base + 3 * sb);
     I.raw = f ? reinterpret_cast<T*>(base + 4 * sb) : nullptr;
     I.part = f ? reinterpret_cast<double*>(base + 4 * sb + rb) : nullptr;
+    I.reg_stride = (long long)(sb / sizeof(T));
     I.nstrips = (p->nx + kTileCols - 1) / kTileCols;
     I.ngroups = (p->ny + kGroupRows - 1) / kGroupRows;
     // Band heights are multiples of the 8-row group; 0 asks for the default cap.
@@ -399,7 +400,7 @@ template <typename T> class Batch : public BatchBase {
   long long pool_bytes_;
   hipStream_t stream_ = nullptr;
   int cus_ = 0, grid_ = 1, nwaves_ = 4;
-  int fill_ = 2;   // tiles per resident wave that the band heights aim for
+  int fill_ = 1;   // tiles per resident wave that the band heights aim for
   BatchInst<T>* d_inst_ = nullptr;
   BatchCtrl* d_ctrl_ = nullptr;
   int* d_meta_ = nullptr;
