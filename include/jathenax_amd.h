@@ -71,6 +71,10 @@ const char* jx_last_error(const jx_ctx* ctx);
 /* Tile shape of the fused stage kernel: block_x threads (64..256, multiple of 64) per
  * strip of block_x-6 columns, `rows` rows per block; 0 selects the default. */
 int jx_set_tiles(jx_ctx* ctx, int block_x, int rows);
+/* The tile shape the next step or run with these parameters will use (the context's
+ * setting, or the occupancy-based default).  No reference counterpart: tile
+ * introspection for the autotuner. */
+int jx_get_tiles(jx_ctx* ctx, const jx_run_params* p, int* block_x, int* rows);
 /* Measurement hooks (no reference counterpart): when on, every fused stage launch is
  * bracketed by HIP events on the context stream; jx_kernel_timing synchronises and
  * returns the number of timed launches and their summed duration, then resets. */

@@ -602,6 +602,17 @@ int jx_set_tiles(jx_ctx* c, int block_x, int rows) {
   return JX_OK;
 }
 
+int jx_get_tiles(jx_ctx* c, const jx_run_params* p, int* block_x, int* rows) {
+  if (!c || !p || !block_x || !rows) return JX_EINVAL;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  const bool visc = p->viscosity > 0.0 || p->smagorinsky_cs > 0.0;
+  const bool exact = is_pow2(p->dx) && is_pow2(p->dy) && is_pow2(p->cs);
+  if (p->dtype == 1) pick_tiles<float>(c, p, visc, exact, block_x, rows);
+  else pick_tiles<double>(c, p, visc, exact, block_x, rows);
+  return JX_OK;
+}
+
 int jx_set_kernel_timing(jx_ctx* c, int on) {
   if (!c) return JX_EINVAL;
   c->timing = on != 0;

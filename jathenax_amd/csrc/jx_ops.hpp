@@ -177,6 +177,35 @@ template <typename T> struct Gas {
   T cs, cs2, inv_cs;
 };
 
+// Roe averages (sqrt(rl) a_l + sqrt(rr) a_r) / (sqrt(rl) + sqrt(rr)) (fluxes.py:145-148,
+// 187-188).  FP64 evaluates the reference expression.  FP32 uses the equal weight
+// sqrt(rr) / (sqrt(rl) + sqrt(rr)) = rr / (sqrt(rl rr) + rr) and a_l + (a_r - a_l) w: one
+// square root and one reciprocal instead of two square roots and a reciprocal (both are
+// quarter-rate instructions).
+template <typename T>
+__device__ __forceinline__ void roe_averages(T rl, T rr, T ul, T ur, T tl, T tr, T& vn, T& vt) {
+  if constexpr (Fast<T>::value) {
+    const T w = rr * dvd(T(1), root(rl * rr) + rr);
+    vn = ul + (ur - ul) * w;
+    vt = tl + (tr - tl) * w;
+  } else {
+    const T sl = root(rl), sr = root(rr);
+    const T inv = dvd(T(1), sl + sr);
+    vn = (sl * ul + sr * ur) * inv;
+    vt = (sl * tl + sr * tr) * inv;
+  }
+}
+template <typename T>
+__device__ __forceinline__ T roe_average(T rl, T rr, T ul, T ur) {
+  if constexpr (Fast<T>::value) {
+    const T w = rr * dvd(T(1), root(rl * rr) + rr);
+    return ul + (ur - ul) * w;
+  } else {
+    const T sl = root(rl), sr = root(rr);
+    return (sl * ul + sr * ur) * dvd(T(1), sl + sr);
+  }
+}
+
 template <int RS, bool EXACT, typename T>
 __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
                                         T& f0, T& f1, T& f2) {
@@ -194,8 +223,7 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     f2 = T(0.5) * (s2 - sp * (mtr - mtl));
   } else if constexpr (RS == RS_HLLE) {
     // fluxes.py:186-201
-    const T sl = root(rl), sr = root(rr);
-    const T vn = (sl * ul + sr * ur) * dvd(T(1), sl + sr);
+    const T vn = roe_average(rl, rr, ul, ur);
     const T al = fmn(vn - cs, ul - cs);
     const T ar = fmx(vn + cs, ur + cs);
     const T bm = (al < T(0)) ? al : T(-1e-20);
@@ -214,10 +242,8 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     const T fl0 = mnl, fl1 = mnl * ul + cs2 * rl, fl2 = mnl * tl;
     const T fr0 = mnr, fr1 = mnr * ur + cs2 * rr, fr2 = mnr * tr;
     const T fa0 = T(0.5) * (fl0 + fr0), fa1 = T(0.5) * (fl1 + fr1), fa2 = T(0.5) * (fl2 + fr2);
-    const T sl = root(rl), sr = root(rr);
-    const T inv = dvd(T(1), sl + sr);
-    const T vn = (sl * ul + sr * ur) * inv;
-    const T vt = (sl * tl + sr * tr) * inv;
+    T vn, vt;
+    roe_averages(rl, rr, ul, ur, tl, tr, vn, vt);
     const T wm = vn - cs, wp = vn + cs;
     const T drho = rr - rl, dmn = mnr - mnl, dmt = mtr - mtl;
     T am, ap;

@@ -92,7 +92,7 @@ __device__ __forceinline__ void block_min_key(unsigned long long k, unsigned lon
 
 template <typename T>
 __host__ __device__ constexpr size_t stage_smem_bytes(int nv, bool visc, int bx) {
-  return (size_t)(3 * nv + (visc ? 9 : 0)) * bx * sizeof(T) + 64 * sizeof(double);
+  return (size_t)(4 * nv + (visc ? 9 : 0)) * bx * sizeof(T) + 64 * sizeof(double);
 }
 
 template <int P> using IC = std::integral_constant<int, P>;
@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   T* sH = sW + NV * BX;                   // [NV][BX] x high edges
   T* sF = sH + NV * BX;                   // [NV][BX] x face fluxes
   T* sV = sF + NV * BX;                   // [3 rows][rho, vx, vy][BX] (viscous only)
-  double* sred = reinterpret_cast<double*>(sF + (NV + (VISC ? 9 : 0)) * BX);
+  T* sI = sF + (NV + (VISC ? 9 : 0)) * BX;   // [NV][BX] PPM4 x interfaces
+  double* sred = reinterpret_cast<double*>(sI + NV * BX);
 
   const int nx = a.nx, ny = a.ny;
   const int OX = BX - 6;
@@ -151,12 +152,16 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
   unsigned long long kmin = ~0ull;
   T pf[NV];     // prefetched conserved values of the next row
+  // Addresses are a wave-uniform 64-bit row base plus a 32-bit per-lane column offset.
+  const size_t row_len = (size_t)nx * NV;
+  const int lane_in = xc * NV;
+  const int lane_out = (x0 + t - 3) * NV;
   {
     int yn = (y0 - 3) % ny;
     if (yn < 0) yn += ny;
-    const T* src = a.u_in + ((size_t)yn * nx + xc) * NV;
+    const T* src = a.u_in + (size_t)yn * row_len;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) pf[v] = src[v];
+    for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
   }
 
   auto body = [&](auto pc, int i) {
@@ -172,9 +177,9 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       if (i + 1 < nit) {
         int yn = (r + 1) % ny;
         if (yn < 0) yn += ny;
-        const T* src = a.u_in + ((size_t)yn * nx + xc) * NV;
+        const T* src = a.u_in + (size_t)yn * row_len;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) pf[v] = src[v];
+        for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
       }
     }
     // Saved register of the row updated this iteration (pointwise, stages >= 2).
@@ -182,9 +187,9 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) svr[v] = T(0);
     if (i >= 6 && !a.stage1) {
-      const T* sv = a.u_sv + ((size_t)(r - 3) * nx + xc) * NV;
+      const T* sv = a.u_sv + (size_t)(r - 3) * row_len;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) svr[v] = sv[v];
+      for (int v = 0; v < NV; ++v) svr[v] = sv[lane_in + v];
     }
     {
       // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
@@ -275,10 +280,29 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     // ---- x direction for row j = r-3, then the update --------------------------------
     if (i >= 6) {
       T xl[NV] = {}, xh[NV] = {};
+      if constexpr (SHARED) {
+        // Each lane evaluates the clamped interface at its cell's low face once; the
+        // high edge is the next lane's interface (same value the reference computes
+        // twice, see ppm4_interface).
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const T* row = sW + v * BX;
-        cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], a.eps, xl[v], xh[v]);
+        for (int v = 0; v < NV; ++v) {
+          const T* row = sW + v * BX;
+          xl[v] = ppm4_interface(row[tm2], row[tm1], row[t], row[tp1]);
+          sI[v * BX + t] = xl[v];
+        }
+        __syncthreads();   // B1b
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          xh[v] = sI[v * BX + tp1];
+          ppm4_limit(sW[v * BX + t], xl[v], xh[v]);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const T* row = sW + v * BX;
+          cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], a.eps, xl[v],
+                            xh[v]);
+        }
       }
       if constexpr (floors_faces<RECON>()) {
         xl[0] = fmx(xl[0], dfloor);
@@ -320,7 +344,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       __syncthreads();   // B3
       if (col_out) {
         const int j = r - 3;
-        const size_t idx = ((size_t)j * nx + (x0 + t - 3)) * NV;
+        T* const out_row = a.u_out + (size_t)j * row_len;
         T un[NV] = {};
         T us[NV] = {};
         if (a.stage1) {
@@ -336,7 +360,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
             us[v] = us[v] + T(a.delta) * u[S3][v];
-            a.s_out[idx + v] = us[v];
+            a.s_out[(size_t)j * row_len + lane_out + v] = us[v];
           }
         }
 #pragma unroll
@@ -355,7 +379,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         }
         if (a.partial) {
           // The kick normalises against the post-drag state; the floor comes after it.
-          const T* rf = a.raw + ((size_t)j * nx + (x0 + t - 3)) * 2;
+          const T* rf = a.raw + (size_t)j * nx * 2 + (x0 + t - 3) * 2;
           acc0 += (double)un[0];
           acc1 += (double)(un[0] * rf[0]);
           acc2 += (double)(un[0] * rf[1]);
@@ -371,7 +395,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           }
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.u_out[idx + v] = un[v];
+        for (int v = 0; v < NV; ++v) out_row[lane_out + v] = un[v];
       }
     }
     if (i >= 5) {

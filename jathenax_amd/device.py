@@ -77,6 +77,13 @@ class Context:
         self.pinned = (int(block_x), int(rows)) != (0, 0)
         self._apply_tiles(block_x, rows)
 
+    def tiles_for(self, p):
+        """(block_x, rows) the next step or run with RunParams p will use."""
+        bx, rows = ctypes.c_int(), ctypes.c_int()
+        self.check(_lib.load().jx_get_tiles(self.handle, ctypes.byref(p), ctypes.byref(bx),
+                                            ctypes.byref(rows)), "get_tiles")
+        return bx.value, rows.value
+
     def _apply_tiles(self, block_x, rows):
         self.check(_lib.load().jx_set_tiles(self.handle, int(block_x), int(rows)), "set_tiles")
 
