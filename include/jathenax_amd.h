@@ -123,6 +123,32 @@ int jx_render_force(jx_ctx* ctx, int dtype, int ny, int nx, int n_modes,
 int jx_normalize_force(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* U,
                        const void* raw_force, double dt, double dedt, void* acc_out);
 
+/* ---- Unfused stage operators --------------------------------------------------------
+ * The traced stage path (isothermal_step(trace=True), hydro.py:134-142) and the
+ * learned-stress path (hydro.py:111-119) run the stage as separate launches that keep
+ * every intermediate array; each entry point restates one reference array function with
+ * the fused kernel's scalar operators.  Arrays are (ny, nx, nvar) unless stated. */
+/* reconstruct_faces (fluxes.py:124-131): left/right states at each cell's low face. */
+int jx_reconstruct_faces(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* W,
+                         int axis, int reconstruct, double dfloor, void* left, void* right);
+/* isothermal_flux (fluxes.py:207-219). */
+int jx_isothermal_flux(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* left,
+                       const void* right, double cs, int axis, int rsolver, void* flux);
+/* viscous_momentum_flux (fluxes.py:248-266): out is (ny, nx, 2). */
+int jx_viscous_momentum_flux(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* W,
+                             double dx, double dy, double viscosity, double smagorinsky_cs,
+                             int axis, void* out);
+/* flux[..., 1:3] += add, add is (ncells, 2) (hydro.py:106-110, 117-119). */
+int jx_add_momentum_flux(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* flux,
+                         const void* add);
+/* Drag in place: U[..., 1:3] -= (beta*dt*drag_rate*rho) * v with the stage's incoming
+ * primitives W (hydro.py:122-125). */
+int jx_apply_drag(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* U, const void* W,
+                  double beta, double dt, double drag_rate);
+/* Kick in place: U[..., 1:3] += rho * (acceleration * dt) (hydro.py:131-132). */
+int jx_apply_kick(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* U,
+                  const void* acceleration, double dt);
+
 /* Float64 box sums of each component (run.py:186-187 before the dx*dy factor). */
 int jx_conserved_sums(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* U,
                       double* sums_out);

@@ -58,6 +58,18 @@ SIGNATURES = {
                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "jx_normalize_force": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                    c_double, c_double, c_void_p]),
+    "jx_reconstruct_faces": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int,
+                                     c_int, c_double, c_void_p, c_void_p]),
+    "jx_isothermal_flux": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                   c_double, c_int, c_int, c_void_p]),
+    "jx_viscous_momentum_flux": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                         c_double, c_double, c_double, c_double, c_int,
+                                         c_void_p]),
+    "jx_add_momentum_flux": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p]),
+    "jx_apply_drag": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p, c_double,
+                              c_double, c_double]),
+    "jx_apply_kick": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p,
+                              c_double]),
     "jx_conserved_sums": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
                                   POINTER(c_double)]),
     "jx_ran_st": (c_int, [c_longlong, c_longlong, POINTER(c_double)]),

@@ -11,6 +11,7 @@
 
 #include "../../include/jathenax_amd.h"
 #include "jx_aux.hpp"
+#include "jx_debug.hpp"
 #include "jx_batch_host.hpp"
 #include "jx_stage.hpp"
 
@@ -545,6 +546,63 @@ int normalize_impl(jx_ctx* c, int ny, int nx, int nvar, const T* U, const T* raw
   return JX_OK;
 }
 
+
+// ---- unfused stage operators (traced and learned-stress stage paths) ------------------
+
+template <typename T>
+int face_states_impl(jx_ctx* c, int ny, int nx, int nv, const T* W, int axis, int recon,
+                     double dfloor, T* L, T* R) {
+  const dim3 g = aux_grid((size_t)nx * ny);
+  const T eps = T(std::max(1e-42, (double)std::numeric_limits<T>::min()));
+#define JX_FACES(RC)                                                                      \
+  hipLaunchKernelGGL((jx::face_states_kernel<T, RC>), g, dim3(jx::kAuxBlock), 0, c->stream, \
+                     W, L, R, nx, ny, nv, axis, T(dfloor), eps)
+  switch (recon) {
+    case jx::RECON_DC: JX_FACES(jx::RECON_DC); break;
+    case jx::RECON_PLM: JX_FACES(jx::RECON_PLM); break;
+    case jx::RECON_PPM4: JX_FACES(jx::RECON_PPM4); break;
+    case jx::RECON_PPMX: JX_FACES(jx::RECON_PPMX); break;
+    case jx::RECON_WENOZ: JX_FACES(jx::RECON_WENOZ); break;
+    default: return fail(c, JX_EKEY, "unknown reconstruct id");
+  }
+#undef JX_FACES
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+template <typename T>
+int riemann_impl(jx_ctx* c, size_t n, int nv, const T* L, const T* R, double cs, int axis,
+                 int rs, T* F) {
+  const dim3 g = aux_grid(n);
+  const jx::Gas<T> gas{T(cs), T(cs * cs), T(1.0 / cs)};
+#define JX_FLUX(RS_)                                                                       \
+  hipLaunchKernelGGL((jx::riemann_faces_kernel<T, RS_>), g, dim3(jx::kAuxBlock), 0,         \
+                     c->stream, L, R, F, n, nv, axis, gas)
+  switch (rs) {
+    case jx::RS_LLF: JX_FLUX(jx::RS_LLF); break;
+    case jx::RS_HLLE: JX_FLUX(jx::RS_HLLE); break;
+    case jx::RS_ROE: JX_FLUX(jx::RS_ROE); break;
+    default: return fail(c, JX_EKEY, "unknown rsolver id");
+  }
+#undef JX_FLUX
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+template <typename T>
+int viscous_impl(jx_ctx* c, int ny, int nx, int nv, const T* W, double dx, double dy,
+                 double nu, double smag_cs, int axis, T* out) {
+  const dim3 g = aux_grid((size_t)nx * ny);
+  const jx::Visc<T> vp{T(nu), T(smag_cs * smag_cs * dx * dy), smag_cs > 0.0};
+  const jx::Over<T, false> odx{T(dx), T(1.0 / dx)}, ody{T(dy), T(1.0 / dy)};
+  const jx::Over<T, false> o4dx{T(4.0 * dx), T(1.0 / (4.0 * dx))};
+  const jx::Over<T, false> o4dy{T(4.0 * dy), T(1.0 / (4.0 * dy))};
+  hipLaunchKernelGGL(jx::viscous_faces_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, W, out,
+                     nx, ny, nv, axis, odx, ody, o4dx, o4dy, vp);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -835,6 +893,107 @@ int jx_conserved_sums(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void
                            c->stream));
   JX_HIP(c, hipStreamSynchronize(c->stream));
   std::memcpy(sums_out, c->host_sums, nvar * sizeof(double));
+  return JX_OK;
+}
+
+// ---- unfused stage operators ------------------------------------------------------------
+
+int jx_reconstruct_faces(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* W,
+                         int axis, int reconstruct, double dfloor, void* left, void* right) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 1 || (axis != 0 && axis != 1) || !W || !left || !right)
+    return fail(c, JX_EINVAL, "bad reconstruct_faces arguments");
+  if (dtype == 0)
+    return face_states_impl<double>(c, ny, nx, nvar, static_cast<const double*>(W), axis,
+                                    reconstruct, dfloor, static_cast<double*>(left),
+                                    static_cast<double*>(right));
+  return face_states_impl<float>(c, ny, nx, nvar, static_cast<const float*>(W), axis,
+                                 reconstruct, dfloor, static_cast<float*>(left),
+                                 static_cast<float*>(right));
+}
+
+int jx_isothermal_flux(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* left,
+                       const void* right, double cs, int axis, int rsolver, void* flux) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 3 || (axis != 0 && axis != 1) || !left || !right || !flux ||
+      !(cs > 0.0))
+    return fail(c, JX_EINVAL, "bad isothermal_flux arguments");
+  const size_t n = (size_t)nx * ny;
+  if (dtype == 0)
+    return riemann_impl<double>(c, n, nvar, static_cast<const double*>(left),
+                                static_cast<const double*>(right), cs, axis, rsolver,
+                                static_cast<double*>(flux));
+  return riemann_impl<float>(c, n, nvar, static_cast<const float*>(left),
+                             static_cast<const float*>(right), cs, axis, rsolver,
+                             static_cast<float*>(flux));
+}
+
+int jx_viscous_momentum_flux(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* W,
+                             double dx, double dy, double viscosity, double smagorinsky_cs,
+                             int axis, void* out) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 3 || (axis != 0 && axis != 1) || !W || !out)
+    return fail(c, JX_EINVAL, "bad viscous_momentum_flux arguments");
+  if (dtype == 0)
+    return viscous_impl<double>(c, ny, nx, nvar, static_cast<const double*>(W), dx, dy,
+                                viscosity, smagorinsky_cs, axis, static_cast<double*>(out));
+  return viscous_impl<float>(c, ny, nx, nvar, static_cast<const float*>(W), dx, dy, viscosity,
+                             smagorinsky_cs, axis, static_cast<float*>(out));
+}
+
+int jx_add_momentum_flux(jx_ctx* c, int dtype, long long ncells, int nvar, void* flux,
+                         const void* add) {
+  if (!c) return JX_EINVAL;
+  if (ncells < 0 || nvar < 3 || !flux || !add)
+    return fail(c, JX_EINVAL, "bad add_momentum_flux arguments");
+  const dim3 g = aux_grid((size_t)ncells);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::add_momentum_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<double*>(flux), static_cast<const double*>(add),
+                       (size_t)ncells, nvar);
+  else
+    hipLaunchKernelGGL(jx::add_momentum_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<float*>(flux), static_cast<const float*>(add),
+                       (size_t)ncells, nvar);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_apply_drag(jx_ctx* c, int dtype, long long ncells, int nvar, void* U, const void* W,
+                  double beta, double dt, double drag_rate) {
+  if (!c) return JX_EINVAL;
+  if (ncells < 0 || nvar < 3 || !U || !W) return fail(c, JX_EINVAL, "bad drag arguments");
+  const dim3 g = aux_grid((size_t)ncells);
+  if (dtype == 0) {
+    const double coef = (beta * dt) * drag_rate;
+    hipLaunchKernelGGL(jx::drag_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<double*>(U), static_cast<const double*>(W), (size_t)ncells,
+                       nvar, coef);
+  } else {
+    const float coef = (float)(beta * dt) * (float)drag_rate;
+    hipLaunchKernelGGL(jx::drag_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<float*>(U), static_cast<const float*>(W), (size_t)ncells,
+                       nvar, coef);
+  }
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+int jx_apply_kick(jx_ctx* c, int dtype, long long ncells, int nvar, void* U,
+                  const void* acceleration, double dt) {
+  if (!c) return JX_EINVAL;
+  if (ncells < 0 || nvar < 3 || !U || !acceleration)
+    return fail(c, JX_EINVAL, "bad kick arguments");
+  const dim3 g = aux_grid((size_t)ncells);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::kick_add_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<double*>(U), static_cast<const double*>(acceleration),
+                       (size_t)ncells, nvar, dt);
+  else
+    hipLaunchKernelGGL(jx::kick_add_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<float*>(U), static_cast<const float*>(acceleration),
+                       (size_t)ncells, nvar, (float)dt);
+  JX_HIP(c, hipGetLastError());
   return JX_OK;
 }
 

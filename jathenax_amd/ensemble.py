@@ -148,7 +148,8 @@ class Ensemble:
         """(U copy, time, last dt, cycle) of an instance."""
         U = torch.empty(self._shapes[iid], dtype=self.dtype, device=self.device)
         t, dt, cyc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-        self._check(_lib.load().jx_batch_result(self.handle, int(iid), ctypes.c_void_p(U.data_ptr()),
+        self._check(_lib.load().jx_batch_result(self.handle, int(iid),
+                                                ctypes.c_void_p(U.data_ptr()),
                                                 ctypes.byref(t), ctypes.byref(dt),
                                                 ctypes.byref(cyc)), "result")
         return U, t.value, dt.value, cyc.value

@@ -45,7 +45,8 @@ def save(path=TABLE_PATH):
 def single_key(p):
     """Key of a RunParams for the single-grid table."""
     visc = int(p.viscosity > 0.0 or p.smagorinsky_cs > 0.0)
-    return f"{p.nx}x{p.ny}x{p.nvar}:{'f32' if p.dtype == 1 else 'f64'}:r{p.reconstruct}s{p.rsolver}v{visc}"
+    prec = "f32" if p.dtype == 1 else "f64"
+    return f"{p.nx}x{p.ny}x{p.nvar}:{prec}:r{p.reconstruct}s{p.rsolver}v{visc}"
 
 
 def apply_single(ctx, p):
