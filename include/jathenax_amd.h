@@ -149,6 +149,25 @@ int jx_apply_drag(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* U, c
 int jx_apply_kick(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* U,
                   const void* acceleration, double dt);
 
+/* ---- Learned subgrid stress in the fused stage (hydro.py:111-119) -------------------
+ * Before each RK stage the library writes the stage input's primitives (recover_isothermal,
+ * hydro.py:54-60) to W (ny, nx, nvar) and calls fn(user, stage, W, tau); the callback must
+ * fill tau (ny, nx, 3) = (tau_xx, tau_xy, tau_yy) with work ordered on the context stream
+ * and return 0 (nonzero aborts with JX_EINVAL).  The fused stage averages tau onto the
+ * faces and adds it to the momentum fluxes after the viscous flux.  W and tau are caller-
+ * owned device buffers of the state dtype. */
+typedef int (*jx_stress_fn)(void* user, int stage, const void* W, void* tau);
+/* isothermal_step(predict_stress=...) (hydro.py:92-146). */
+int jx_isothermal_step_sgs(jx_ctx* ctx, const jx_run_params* p, const void* U_in, void* U_out,
+                           double dt, const void* raw_force, double dedt, void* W, void* tau,
+                           jx_stress_fn fn, void* user);
+/* evolve_isothermal(predict_stress=...) (run.py:20-69); the device loop, one step per
+ * host round trip so that the callback runs between stages. */
+int jx_evolve_isothermal_sgs(jx_ctx* ctx, const jx_run_params* p, const void* U_in,
+                             void* U_out, const jx_forcing* f, void* W, void* tau,
+                             jx_stress_fn fn, void* user, double* time_out, double* dt_out,
+                             int* cycle_out);
+
 /* Float64 box sums of each component (run.py:186-187 before the dx*dy factor). */
 int jx_conserved_sums(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* U,
                       double* sums_out);

@@ -30,6 +30,9 @@ class Forcing(ctypes.Structure):
     ]
 
 
+# jx_stress_fn: int (*)(void* user, int stage, const void* W, void* tau).
+STRESS_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_int, c_void_p, c_void_p)
+
 # name -> (restype, argtypes); every symbol the header declares.
 SIGNATURES = {
     "jx_ctx_create": (c_int, [POINTER(c_void_p), c_int, c_void_p]),
@@ -41,6 +44,13 @@ SIGNATURES = {
     "jx_kernel_timing": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double)]),
     "jx_isothermal_step": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p, c_double,
                                    c_void_p, c_double]),
+    "jx_isothermal_step_sgs": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,
+                                       c_double, c_void_p, c_double, c_void_p, c_void_p,
+                                       STRESS_FN, c_void_p]),
+    "jx_evolve_isothermal_sgs": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,
+                                         POINTER(Forcing), c_void_p, c_void_p, STRESS_FN,
+                                         c_void_p, POINTER(c_double), POINTER(c_double),
+                                         POINTER(c_int)]),
     "jx_isothermal_timestep": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_double,
                                        c_double, c_double, POINTER(c_double)]),
     "jx_evolve_isothermal": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,

@@ -236,10 +236,22 @@ template <typename T> struct StepPlan {
   StageArgs<T> base;
 };
 
-template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p) {
+// Learned-stress hook of a step: before each stage the stage input's primitives are
+// written to W and fn fills tau (ny, nx, 3) on the context stream.
+template <typename T> struct SgsHook {
+  T* W;
+  T* tau;
+  jx_stress_fn fn;
+  void* user;
+};
+
+// `sgs`: the stages add a learned stress, which only the viscous kernel variants read.
+template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p,
+                                            bool sgs = false) {
   StepPlan<T> s;
   s.p = p;
-  s.visc = p->viscosity > 0.0 || p->smagorinsky_cs > 0.0;
+  const bool visc_terms = p->viscosity > 0.0 || p->smagorinsky_cs > 0.0;
+  s.visc = visc_terms || sgs;
   const bool exact = is_pow2(p->dx) && is_pow2(p->dy) && is_pow2(p->cs);
   pick_tiles<T>(c, p, s.visc, exact, &s.bx, &s.rows);
   s.grid = dim3((unsigned)((p->nx + (s.bx - 6) - 1) / (s.bx - 6)),
@@ -268,6 +280,7 @@ template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p) {
   a.smag_c = T(p->smagorinsky_cs * p->smagorinsky_cs * p->dx * p->dy);
   a.drag_on = p->drag_rate > 0.0;
   a.drag = T(p->drag_rate);
+  a.visc_terms = visc_terms ? 1 : 0;
   s.base = a;
   return s;
 }
@@ -276,7 +289,7 @@ template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p) {
 // (floored on read).  The result lands in `A`.  `raw`: kick the final stage.
 template <typename T>
 int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
-                const T* raw, bool want_dtkey) {
+                const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr) {
   const jx_run_params* p = s.p;
   const RkRow* rows = rk_rows(p->integrator);
   const int ns = p->integrator;
@@ -333,6 +346,17 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
       a.partial = c->partial;
     } else if (last && want_dtkey) {
       a.dtkey = &c->ctrl->dtkey;
+    }
+    if (hook) {
+      // recover_isothermal of the stage input, then the caller's stress (hydro.py:111-115).
+      const size_t ncell = (size_t)p->nx * p->ny;
+      hipLaunchKernelGGL(jx::recover_kernel<T>, aux_grid(ncell), dim3(jx::kAuxBlock), 0,
+                         c->stream, a.u_in, static_cast<T*>(nullptr), hook->W, ncell, p->nvar,
+                         T(p->dfloor));
+      JX_HIP(c, hipGetLastError());
+      if (hook->fn(hook->user, k, hook->W, hook->tau) != 0)
+        return fail(c, JX_EINVAL, "the stress callback failed");
+      a.tau = hook->tau;
     }
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
     if (c->timing) {
@@ -404,18 +428,18 @@ jx::TimestepArgs<T> timestep_args(jx_ctx* c, const jx_run_params* p, const T* q,
 
 template <typename T>
 int step_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, double dt, const T* raw,
-              double dedt) {
+              double dedt, const SgsHook<T>* hook = nullptr) {
   const size_t bytes = (size_t)p->nx * p->ny * p->nvar * sizeof(T);
   int rc = ensure_buffers(c, bytes);
   if (rc) return rc;
-  StepPlan<T> s = make_plan<T>(c, p);
+  StepPlan<T> s = make_plan<T>(c, p, hook != nullptr);
   hipLaunchKernelGGL(jx::set_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, dt, dedt);
-  return launch_step<T>(c, s, in, out, true, raw, false);
+  return launch_step<T>(c, s, in, out, true, raw, false, hook);
 }
 
 template <typename T>
 int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx_forcing* f,
-                double* t_out, double* dt_out, int* cyc_out) {
+                double* t_out, double* dt_out, int* cyc_out, const SgsHook<T>* hook = nullptr) {
   const size_t n = (size_t)p->nx * p->ny;
   const size_t bytes = n * p->nvar * sizeof(T);
   int rc = ensure_buffers(c, bytes);
@@ -434,7 +458,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     *cyc_out = 0;
     return JX_OK;
   }
-  StepPlan<T> s = make_plan<T>(c, p);
+  StepPlan<T> s = make_plan<T>(c, p, hook != nullptr);
   const bool smag = p->smagorinsky_cs > 0.0;
   jx::LoopArgs la{};
   la.tlim = p->tlim;
@@ -473,7 +497,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     if (forced)
       hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
                          c->stream, ra);
-    int r = launch_step<T>(c, s, in, out, first, raw, !smag);
+    int r = launch_step<T>(c, s, in, out, first, raw, !smag, hook);
     if (r) return r;
     if (smag)
       hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
@@ -500,12 +524,14 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     double est = (p->tlim - h.time) / h.dt;
     long long want = std::isfinite(est) ? (long long)std::ceil(est) : 1;
     want = std::min<long long>(want, (long long)p->nlim - h.cycle);
-    chunk = std::max<long long>(1, std::min<long long>(want, 512));
+    // With a stress callback every step is its own chunk: the callback is host code, and
+    // steps past the end must not call it.
+    chunk = hook ? 1 : std::max<long long>(1, std::min<long long>(want, 512));
     {
       if (forced)
         hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
                            c->stream, ra);
-      int r = launch_step<T>(c, s, in, out, false, raw, !smag);
+      int r = launch_step<T>(c, s, in, out, false, raw, !smag, hook);
       if (r) return r;
       if (smag)
         hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
@@ -698,16 +724,27 @@ int jx_kernel_timing(jx_ctx* c, int* launches, double* total_ms) {
 
 int jx_isothermal_step(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
                        double dt, const void* raw_force, double dedt) {
+  return jx_isothermal_step_sgs(c, p, U_in, U_out, dt, raw_force, dedt, nullptr, nullptr,
+                                nullptr, nullptr);
+}
+
+int jx_isothermal_step_sgs(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
+                           double dt, const void* raw_force, double dedt, void* W, void* tau,
+                           jx_stress_fn fn, void* user) {
   if (!c) return JX_EINVAL;
   int rc = check_params(c, p);
   if (rc) return rc;
   if (!U_in || !U_out || U_in == U_out)
     return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
-  if (p->dtype == 0)
+  if (fn && (!W || !tau)) return fail(c, JX_EINVAL, "a stress callback needs W and tau buffers");
+  if (p->dtype == 0) {
+    SgsHook<double> h{static_cast<double*>(W), static_cast<double*>(tau), fn, user};
     return step_impl<double>(c, p, static_cast<const double*>(U_in), static_cast<double*>(U_out),
-                             dt, static_cast<const double*>(raw_force), dedt);
+                             dt, static_cast<const double*>(raw_force), dedt, fn ? &h : nullptr);
+  }
+  SgsHook<float> h{static_cast<float*>(W), static_cast<float*>(tau), fn, user};
   return step_impl<float>(c, p, static_cast<const float*>(U_in), static_cast<float*>(U_out), dt,
-                          static_cast<const float*>(raw_force), dedt);
+                          static_cast<const float*>(raw_force), dedt, fn ? &h : nullptr);
 }
 
 int jx_isothermal_timestep(jx_ctx* c, const jx_run_params* p, const void* W, double previous_dt,
@@ -738,7 +775,15 @@ int jx_isothermal_timestep(jx_ctx* c, const jx_run_params* p, const void* W, dou
 
 int jx_evolve_isothermal(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
                          const jx_forcing* f, double* time_out, double* dt_out, int* cycle_out) {
+  return jx_evolve_isothermal_sgs(c, p, U_in, U_out, f, nullptr, nullptr, nullptr, nullptr,
+                                  time_out, dt_out, cycle_out);
+}
+
+int jx_evolve_isothermal_sgs(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
+                             const jx_forcing* f, void* W, void* tau, jx_stress_fn fn,
+                             void* user, double* time_out, double* dt_out, int* cycle_out) {
   if (!c || !time_out || !dt_out || !cycle_out) return JX_EINVAL;
+  if (fn && (!W || !tau)) return fail(c, JX_EINVAL, "a stress callback needs W and tau buffers");
   int rc = check_params(c, p);
   if (rc) return rc;
   if (!std::isfinite(p->dedt) || p->dedt < 0.0)
@@ -751,11 +796,15 @@ int jx_evolve_isothermal(jx_ctx* c, const jx_run_params* p, const void* U_in, vo
     return fail(c, JX_EINVAL, "incomplete forcing tables");
   if (!U_in || !U_out || U_in == U_out)
     return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
-  if (p->dtype == 0)
+  if (p->dtype == 0) {
+    SgsHook<double> h{static_cast<double*>(W), static_cast<double*>(tau), fn, user};
     return evolve_impl<double>(c, p, static_cast<const double*>(U_in),
-                               static_cast<double*>(U_out), f, time_out, dt_out, cycle_out);
+                               static_cast<double*>(U_out), f, time_out, dt_out, cycle_out,
+                               fn ? &h : nullptr);
+  }
+  SgsHook<float> h{static_cast<float*>(W), static_cast<float*>(tau), fn, user};
   return evolve_impl<float>(c, p, static_cast<const float*>(U_in), static_cast<float*>(U_out), f,
-                            time_out, dt_out, cycle_out);
+                            time_out, dt_out, cycle_out, fn ? &h : nullptr);
 }
 
 int jx_recover_isothermal(jx_ctx* c, int dtype, long long ncells, int nvar, const void* U,

@@ -30,6 +30,8 @@ template <typename T> struct StageArgs {
   T* u_out;                     // stage output
   T* s_out;                     // RK4 saved register output (nullptr when no delta)
   const T* __restrict__ raw;    // raw force (ny, nx, 2) when this stage feeds the kick
+  const T* __restrict__ tau;    // learned stress (ny, nx, 3) of this stage, or nullptr
+                                // (VISC kernels only)
   const Ctrl* ctrl;             // dt of the step and the done flag
   unsigned long long* dtkey;    // CFL-limit min slot (nullptr: no epilogue)
   double* partial;              // kick moments per block [3] (nullptr: no kick)
@@ -37,6 +39,7 @@ template <typename T> struct StageArgs {
   int stage1;                   // saved register is the (floored) stage input itself
   int floor_saved;              // apply the density floor when reading u_sv
   int has_delta, blended, drag_on, smag_on;
+  int visc_terms;               // VISC kernels: add the viscous flux (off: stress only)
   int exact;                    // dx, dy, cs are powers of two
   T dx, dy, cs, cs2, inv_cs, dfloor, eps, four_dx, four_dy, visc, smag_c, drag;
   T inv_dx, inv_dy, inv_four_dx, inv_four_dy;
@@ -225,7 +228,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         gn[2] = f1;
 #pragma unroll
         for (int v = 3; v < NV; ++v) gn[v] = f0 * ((f0 >= T(0)) ? hy[v] : lo[v]);
-        if constexpr (VISC) {
+        if (VISC && a.visc_terms) {
           // low y face of row s = r-2, column xc (fluxes.py:227-238 with axis=0)
           const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row s
           const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row s-1
@@ -238,6 +241,17 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           visc_face<0>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
           gn[1] = gn[1] + fmx_;
           gn[2] = gn[2] + fmy_;
+        }
+        if (VISC && a.tau) {
+          // Learned stress averaged onto the low y face of row r-2, added after the
+          // viscous flux (hydro.py:111-119, fluxes.py:269-280): (tau_xy, tau_yy).
+          int s0 = (r - 2) % ny, sm = (r - 3) % ny;
+          if (s0 < 0) s0 += ny;
+          if (sm < 0) sm += ny;
+          const T* t0 = a.tau + (size_t)s0 * nx * 3 + xc * 3;
+          const T* tm = a.tau + (size_t)sm * nx * 3 + xc * 3;
+          gn[1] = gn[1] + T(0.5) * (tm[1] + t0[1]);
+          gn[2] = gn[2] + T(0.5) * (tm[2] + t0[2]);
         }
       }
 #pragma unroll
@@ -323,7 +337,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         fx[2] = f2;
 #pragma unroll
         for (int v = 3; v < NV; ++v) fx[v] = f0 * ((f0 >= T(0)) ? l[v] : xl[v]);
-        if constexpr (VISC) {
+        if (VISC && a.visc_terms) {
           // low x face of row j = r-3 at column xc (fluxes.py:227-238 with axis=1)
           const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row j
           const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row j+1
@@ -337,6 +351,13 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           visc_face<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
           fx[1] = fx[1] + fmx_;
           fx[2] = fx[2] + fmy_;
+        }
+        if (VISC && a.tau) {
+          // Learned stress on the low x face of column xc of row r-3: (tau_xx, tau_xy).
+          const int xm = (xc == 0) ? nx - 1 : xc - 1;
+          const T* trow = a.tau + (size_t)(r - 3) * nx * 3;
+          fx[1] = fx[1] + T(0.5) * (trow[xm * 3 + 0] + trow[xc * 3 + 0]);
+          fx[2] = fx[2] + T(0.5) * (trow[xm * 3 + 1] + trow[xc * 3 + 1]);
         }
       }
 #pragma unroll

@@ -100,8 +100,13 @@ def isothermal_step(U, dt, dx, dy, cs, dfloor, integrator, *,
                     viscosity=0.0, smagorinsky_cs=0.0, drag_rate=0.0,
                     raw_force=None, dedt=0.0, trace=False, reconstruct="ppm4", rsolver="roe",
                     predict_stress=None, sgs_weights=None):
-    """Advance one periodic isothermal step; every RK stage is one fused HIP launch."""
-    if trace or predict_stress is not None:
+    """Advance one periodic isothermal step; every RK stage is one fused HIP launch.
+
+    trace=True runs the stages as separate launches and returns (U, stages) with the
+    reference's dictionaries.  predict_stress(sgs_weights, primitives) is called before
+    each stage with the stage's (rho, vx, vy) as a (ny, nx, 3) tensor view and must return
+    a floating (ny, nx, 3) stress, which the fused stage adds to the momentum fluxes."""
+    if trace:
         from .trace import traced_step
         return traced_step(U, dt, dx, dy, cs, dfloor, integrator, viscosity=viscosity,
                            smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate,
@@ -120,9 +125,21 @@ def isothermal_step(U, dt, dx, dy, cs, dfloor, integrator, *,
     ctx = context_for(U)
     from . import tuning
     tuning.apply_single(ctx, p)
-    ctx.check(_lib.load().jx_isothermal_step(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
-                                             float(dt), ptr(raw), float(dedt)),
-              "isothermal_step")
+    if predict_stress is None:
+        ctx.check(_lib.load().jx_isothermal_step(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
+                                                 float(dt), ptr(raw), float(dedt)),
+                  "isothermal_step")
+        return out
+    from .trace import stress_callback
+    W = torch.empty_like(U)
+    tau = torch.empty((U.shape[0], U.shape[1], 3), dtype=U.dtype, device=U.device)
+    fn, errors = stress_callback(predict_stress, sgs_weights, W, tau)
+    rc = _lib.load().jx_isothermal_step_sgs(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
+                                            float(dt), ptr(raw), float(dedt), ptr(W), ptr(tau),
+                                            fn, None)
+    if errors:
+        raise errors[0]
+    ctx.check(rc, "isothermal_step")
     return out
 
 

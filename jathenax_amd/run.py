@@ -35,22 +35,17 @@ def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
             raise ValueError(f"{name} must be finite and nonnegative")
     if forcing is not None and (not 0.0 < dt_update < np.inf or not math.isfinite(tdriv_start)):
         raise ValueError("dt_update must be finite and positive and tdriv_start finite")
-    if predict_stress is not None:
-        from .trace import evolve_with_predictor
-        return evolve_with_predictor(
-            U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, viscosity=viscosity,
-            smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
-            dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt, reconstruct=reconstruct,
-            rsolver=rsolver, predict_stress=predict_stress, sgs_weights=sgs_weights)
     return _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, viscosity=viscosity,
                    smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
                    dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt,
-                   reconstruct=reconstruct, rsolver=rsolver, apply_table=True)
+                   reconstruct=reconstruct, rsolver=rsolver, apply_table=True,
+                   predict_stress=predict_stress, sgs_weights=sgs_weights)
 
 
 def _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, viscosity=0.0,
             smagorinsky_cs=0.0, drag_rate=0.0, forcing=None, dt_update=1.0, tdriv_start=0.0,
-            dedt=0.0, reconstruct="ppm4", rsolver="roe", apply_table=True):
+            dedt=0.0, reconstruct="ppm4", rsolver="roe", apply_table=True,
+            predict_stress=None, sgs_weights=None):
     U = as_device(U)
     p = run_params(U, dx, dy, cs, dfloor, integrator, reconstruct=reconstruct, rsolver=rsolver,
                    viscosity=viscosity, smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate,
@@ -77,9 +72,23 @@ def _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, viscosity=0.0
     ctx = context_for(U)
     if apply_table:
         tuning.apply_single(ctx, p)
-    ctx.check(_lib.load().jx_evolve_isothermal(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
-                                               fptr, ctypes.byref(t), ctypes.byref(dt),
-                                               ctypes.byref(cyc)), "evolve_isothermal")
+    if predict_stress is None:
+        ctx.check(_lib.load().jx_evolve_isothermal(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
+                                                   fptr, ctypes.byref(t), ctypes.byref(dt),
+                                                   ctypes.byref(cyc)), "evolve_isothermal")
+    else:
+        # The device loop with a stress callback between stages (one step per round trip).
+        from .trace import stress_callback
+        W = torch.empty_like(U)
+        tau = torch.empty((U.shape[0], U.shape[1], 3), dtype=U.dtype, device=U.device)
+        fn, errors = stress_callback(predict_stress, sgs_weights, W, tau)
+        rc = _lib.load().jx_evolve_isothermal_sgs(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
+                                                  fptr, ptr(W), ptr(tau), fn, None,
+                                                  ctypes.byref(t), ctypes.byref(dt),
+                                                  ctypes.byref(cyc))
+        if errors:
+            raise errors[0]
+        ctx.check(rc, "evolve_isothermal")
     del keep
     return out, t.value, dt.value, cyc.value
 

@@ -47,14 +47,36 @@ def _kick(U, acceleration, dt):
     return out
 
 
-def _stress_faces(predict_stress, sgs_weights, W, dtype):
+def checked_stress(predict_stress, sgs_weights, W, dtype):
+    """The caller's stress for primitives W[..., :3], validated as in hydro.py:113-115."""
     primitives = W[..., :3]
     tau = predict_stress(sgs_weights, primitives)
     if not isinstance(tau, torch.Tensor):
         tau = torch.as_tensor(np.asarray(tau))
     if tuple(tau.shape) != tuple(primitives.shape) or not tau.is_floating_point():
         raise ValueError("predict_stress must return a floating array of shape (ny, nx, 3)")
-    return stress_face_fluxes(tau.to(device=W.device, dtype=dtype))
+    return tau.to(device=W.device, dtype=dtype)
+
+
+def stress_callback(predict_stress, sgs_weights, W, tau):
+    """A jx_stress_fn for the fused stage: W and tau are the device buffers handed to the
+    library; exceptions are kept (the C side only sees a nonzero status) and re-raised by
+    the caller."""
+    errors = []
+
+    def call(user, stage, W_ptr, tau_ptr):
+        try:
+            tau.copy_(checked_stress(predict_stress, sgs_weights, W, tau.dtype))
+            return 0
+        except BaseException as e:   # re-raised after the C call returns
+            errors.append(e)
+            return 1
+
+    return _lib.STRESS_FN(call), errors
+
+
+def _stress_faces(predict_stress, sgs_weights, W, dtype):
+    return stress_face_fluxes(checked_stress(predict_stress, sgs_weights, W, dtype))
 
 
 def traced_step(U, dt, dx, dy, cs, dfloor, integrator, *, viscosity=0.0, smagorinsky_cs=0.0,

@@ -135,14 +135,22 @@ def test_traced_fp32_tracks_fused_fp32(jx):
     assert _rel(_np(traced), fused) <= 1e-6
 
 
-def test_predict_stress_step_matches_oracle(jx):
-    U = _state(40, 32, 4, 7)
-    args = (0.004, 1 / 32, 1 / 40, 1.0, 1e-6, "rk3")
-    kw = dict(reconstruct="wenoz", rsolver="roe", viscosity=1e-3)
+@pytest.mark.parametrize("recon,rs,integ,nv,phys", [
+    ("wenoz", "roe", "rk3", 4, dict(viscosity=1e-3)),
+    ("ppm4", "hlle", "rk2", 3, {}),
+    ("plm", "llf", "rk4", 5, dict(smagorinsky_cs=0.17, drag_rate=0.4)),
+])
+def test_predict_stress_step_matches_oracle(jx, recon, rs, integ, nv, phys):
+    U = _state(40, 32, nv, 7)
+    args = (0.004, 1 / 32, 1 / 40, 1.0, 1e-6, integ)
+    kw = dict(reconstruct=recon, rsolver=rs, **phys)
     ref = ox.isothermal_step(U, *args, predict_stress=_stress_np, sgs_weights=0.05, **kw)
-    got = jx.isothermal_step(torch.tensor(U, device="cuda"), *args,
-                             predict_stress=_stress_torch, sgs_weights=0.05, **kw)
-    assert _rel(_np(got), ref) <= 1e-13
+    Ud = torch.tensor(U, device="cuda")
+    fused = jx.isothermal_step(Ud, *args, predict_stress=_stress_torch, sgs_weights=0.05, **kw)
+    assert _rel(_np(fused), ref) <= 1e-13
+    traced, _ = jx.isothermal_step(Ud, *args, predict_stress=_stress_torch, sgs_weights=0.05,
+                                   trace=True, **kw)
+    np.testing.assert_array_equal(_np(fused), _np(traced))
     plain = ox.isothermal_step(U, *args, **kw)
     assert _rel(ref, plain) > 1e-8   # the stress is not negligible
 
@@ -155,6 +163,9 @@ def test_predict_stress_contract(jx):
     bad_type = lambda w, prim: torch.zeros((16, 16, 3), dtype=torch.int32, device=prim.device)
     with pytest.raises(ValueError):
         jx.isothermal_step(U, 0.001, 1 / 16, 1 / 16, 1.0, 1e-6, "rk2", predict_stress=bad_type)
+    with pytest.raises(ValueError):
+        jx.isothermal_step(U, 0.001, 1 / 16, 1 / 16, 1.0, 1e-6, "rk2", predict_stress=bad_shape,
+                           trace=True)
     numpy_out = lambda w, prim: np.zeros((16, 16, 3))
     out = jx.isothermal_step(U, 0.001, 1 / 16, 1 / 16, 1.0, 1e-6, "rk2",
                              predict_stress=numpy_out)
@@ -173,8 +184,7 @@ def test_evolve_with_predictor(jx):
     kw = dict(viscosity=1e-3, smagorinsky_cs=0.17, drag_rate=0.3, dt_update=0.002,
               tdriv_start=0.0, dedt=0.5, reconstruct="ppm4", rsolver="hlle")
     args = (1 / nx, 1 / ny, 1.0, 1e-6, "rk2", 0.4, 0.05, 12)
-    # A zero stress reproduces the fused device loop: bit for bit unforced; forced, the
-    # kick's box sums are reduced in a different fixed order.
+    # A zero stress reproduces the fused device loop bit for bit.
     zero = lambda w, prim: torch.zeros_like(prim)
     unforced = {k: v for k, v in kw.items() if k not in ("dt_update", "tdriv_start", "dedt")}
     got = jx.evolve_isothermal(torch.tensor(U, device="cuda"), *args, predict_stress=zero,
@@ -186,7 +196,8 @@ def test_evolve_with_predictor(jx):
                                predict_stress=zero, **kw)
     fused = jx.evolve_isothermal(torch.tensor(U, device="cuda"), *args, forcing=(coef, basis),
                                  **kw)
-    assert got[3] == fused[3] and _rel(_np(got[0]), _np(fused[0])) <= 1e-12
+    np.testing.assert_array_equal(_np(got[0]), _np(fused[0]))
+    assert got[1:] == fused[1:]
     # A nonzero stress against the oracle's loop.
     coef_np = np.asarray(coef.cpu() if isinstance(coef, torch.Tensor) else coef)
     basis_np = tuple(np.asarray(b.cpu() if isinstance(b, torch.Tensor) else b) for b in basis)
@@ -196,3 +207,8 @@ def test_evolve_with_predictor(jx):
                                predict_stress=_stress_torch, sgs_weights=0.05, **kw)
     assert got[3] == ref[3] and abs(got[1] - ref[1]) <= 1e-12 * ref[1]
     assert _rel(_np(got[0]), ref[0]) <= 1e-12
+    # The unfused host loop (one traced step per cycle) agrees with the fused loop.
+    from jathenax_amd.trace import evolve_with_predictor
+    unf = evolve_with_predictor(torch.tensor(U, device="cuda"), *args, forcing=(coef, basis),
+                                predict_stress=_stress_torch, sgs_weights=0.05, **kw)
+    assert unf[3] == got[3] and _rel(_np(unf[0]), _np(got[0])) <= 1e-12
