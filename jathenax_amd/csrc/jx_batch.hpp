@@ -34,6 +34,7 @@ namespace jx {
 constexpr int kBatchNV = 4;            // variables per instance: hydro + one scalar
 constexpr int kBatchBlock = 256;       // four wave workers per block
 constexpr int kTileCols = 64 - 6;      // output columns per strip
+constexpr int kBatchSlice = 4 * kBatchNV + 9;   // LDS rows of 64 values per wave worker
 
 template <typename T> struct BatchInst {
   T* A;                  // state between steps (floored after every step)
@@ -219,18 +220,23 @@ __device__ __forceinline__ T over_rt(T x, T d, T inv) {
 
 // One RK stage of one instance (NV variables, reconstruction RECON) over one tile, executed
 // by one wave.  Same arithmetic as stage_kernel; `lds` is this wave's slice
-// (3 * kBatchNV + 9 rows of 64).  The Riemann solver and the physics switches stay runtime
+// (kBatchSlice rows of 64).  The Riemann solver and the physics switches stay runtime
 // choices (uniform branches); the reconstruction is compiled in, so that the LDS reads and
 // the stencil arithmetic of all variables can be scheduled together.
 template <typename T, int NV, int RECON>
 __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtrl& C,
                                                  const Tile& tl, T* lds) {
-  constexpr int R = 5;
+  // PPM4 shares each clamped interface between the two cells it bounds (in y through a
+  // 4-row register ring, in x through the sI slice); the other methods need the full
+  // 5-row stencil.
+  constexpr bool SHARED = RECON == RECON_PPM4;
+  constexpr int R = SHARED ? 4 : 5;
   const int t = threadIdx.x & 63;
   T* sW = lds;             // [NV][64]
   T* sH = sW + NV * 64;    // [NV][64]
   T* sF = sH + NV * 64;    // [NV][64]
   T* sV = sF + NV * 64;    // [3][3][64]
+  T* sI = sV + 9 * 64;     // [NV][64] PPM4 x interfaces
   const int nx = I.nx, ny = I.ny;
   constexpr int nv = NV;
   const int x0 = tl.x0;
@@ -287,10 +293,10 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
   const bool visc_on = I.visc_on != 0;
   constexpr bool floors = RECON == RECON_PPMX || RECON == RECON_WENOZ;
 
-  T w[R][NV], u[R][NV], hy[NV], gp[NV], gn[NV], pf[NV];
+  T w[R][NV], u[R][NV], hy[NV], gp[NV], gn[NV], pf[NV], ip[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    hy[v] = gp[v] = gn[v] = pf[v] = T(0);
+    hy[v] = gp[v] = gn[v] = pf[v] = ip[v] = T(0);
 #pragma unroll
     for (int q = 0; q < R; ++q) w[q][v] = u[q][v] = T(0);
   }
@@ -349,23 +355,16 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     }
     wave_sync();
 
-    if (i >= 4) {
-      T lo[NV], hi[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-        cell_edges<RECON>(w[S4][v], w[S3][v], w[S2][v], w[S1][v], w[S0][v], I.eps, lo[v],
-                    hi[v]);
-      if (floors) {
-        lo[0] = fmx(lo[0], dfloor);
-        hi[0] = fmx(hi[0], dfloor);
-      }
+    // y direction: edges of row r-2, flux at its low face.
+    auto y_face = [&](T (&lo)[NV], T (&hi)[NV]) {
       if (i >= 5) {
         T f0, f1, f2;
         riemann_rt<T>(I.rs, hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
         gn[0] = f0;
         gn[1] = f2;
         gn[2] = f1;
-        gn[3] = f0 * ((f0 >= T(0)) ? hy[3] : lo[3]);
+#pragma unroll
+        for (int v = 3; v < NV; ++v) gn[v] = f0 * ((f0 >= T(0)) ? hy[v] : lo[v]);
         if (visc_on) {
           const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * 64;
           const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * 64;
@@ -383,14 +382,63 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) hy[v] = hi[v];
+    };
+    if constexpr (SHARED) {
+      if (i >= 3) {
+        T in[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          in[v] = ppm4_interface(w[S3][v], w[S2][v], w[S1][v], w[S0][v]);   // below row r-1
+        if (i >= 4) {
+          T lo[NV], hi[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            lo[v] = ip[v];
+            hi[v] = in[v];
+            ppm4_limit(w[S2][v], lo[v], hi[v]);
+          }
+          y_face(lo, hi);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ip[v] = in[v];
+      }
+    } else {
+      if (i >= 4) {
+        T lo[NV], hi[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          cell_edges<RECON>(w[S4][v], w[S3][v], w[S2][v], w[S1][v], w[S0][v], I.eps, lo[v],
+                            hi[v]);
+        if (floors) {
+          lo[0] = fmx(lo[0], dfloor);
+          hi[0] = fmx(hi[0], dfloor);
+        }
+        y_face(lo, hi);
+      }
     }
 
     if (i >= 6) {
       T xl[NV], xh[NV];
+      if constexpr (SHARED) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const T* row = sW + v * 64;
-        cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], I.eps, xl[v], xh[v]);
+        for (int v = 0; v < NV; ++v) {
+          const T* row = sW + v * 64;
+          xl[v] = ppm4_interface(row[tm2], row[tm1], row[t], row[tp1]);
+          sI[v * 64 + t] = xl[v];
+        }
+        wave_sync();
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          xh[v] = sI[v * 64 + tp1];
+          ppm4_limit(sW[v * 64 + t], xl[v], xh[v]);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const T* row = sW + v * 64;
+          cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], I.eps, xl[v],
+                            xh[v]);
+        }
       }
       if (floors) {
         xl[0] = fmx(xl[0], dfloor);
@@ -409,7 +457,8 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
         fx[0] = f0;
         fx[1] = f1;
         fx[2] = f2;
-        fx[3] = f0 * ((f0 >= T(0)) ? l[3] : xl[3]);
+#pragma unroll
+        for (int v = 3; v < NV; ++v) fx[v] = f0 * ((f0 >= T(0)) ? l[v] : xl[v]);
         if (visc_on) {
           const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * 64;
           const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * 64;
@@ -501,7 +550,9 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     if (i0 + 1 < nit) body(IC<1>{}, i0 + 1);
     if (i0 + 2 < nit) body(IC<2>{}, i0 + 2);
     if (i0 + 3 < nit) body(IC<3>{}, i0 + 3);
-    if (i0 + 4 < nit) body(IC<4>{}, i0 + 4);
+    if constexpr (R == 5) {
+      if (i0 + 4 < nit) body(IC<4 % R>{}, i0 + 4);
+    }
   }
   if (want_key) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -529,9 +580,9 @@ __device__ __forceinline__ void batch_stage_dispatch(const BatchInst<T>& I, Batc
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBatchBlock) batch_stage_kernel(BatchTabs<T> b) {
+__global__ void __launch_bounds__(kBatchBlock, 2) batch_stage_kernel(BatchTabs<T> b) {
   extern __shared__ double bsm[];
-  T* lds = reinterpret_cast<T*>(bsm) + (threadIdx.x >> 6) * (3 * kBatchNV + 9) * 64;
+  T* lds = reinterpret_cast<T*>(bsm) + (threadIdx.x >> 6) * kBatchSlice * 64;
   const int* L = b.lists + WL_STAGE * list_stride(b.cap);
   const int total = ((const cmem_t<int>*)L)[1];
   for (int k = wave_first(); k < total; k = wave_then(&b.counters[CNT_STAGE])) {

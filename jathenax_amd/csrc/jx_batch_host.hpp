@@ -306,7 +306,7 @@ template <typename T> class Batch : public BatchBase {
     return (size_t)3 * sizeof(double) * ((nx + kTileCols - 1) / kTileCols) *
            ((ny + kGroupRows - 1) / kGroupRows);
   }
-  static constexpr size_t smem() { return (size_t)4 * (3 * kBatchNV + 9) * 64 * sizeof(T); }
+  static constexpr size_t smem() { return (size_t)4 * kBatchSlice * 64 * sizeof(T); }
 
   int fail(const std::string& m, int code = JX_EINVAL) {
     err = m;

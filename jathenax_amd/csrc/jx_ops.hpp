@@ -247,7 +247,12 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     const T wm = vn - cs, wp = vn + cs;
     const T drho = rr - rl, dmn = mnr - mnl, dmt = mtr - mtl;
     T am, ap;
-    if constexpr (Fast<T>::value || EXACT) {
+    if constexpr (Fast<T>::value) {
+      // am + ap = drho exactly, so FP32 forms ap from am; 0.5/cs is folded into one factor.
+      const T h = T(0.5) * g.inv_cs;
+      am = drho * (T(0.5) + vn * h) - dmn * h;
+      ap = drho - am;
+    } else if constexpr (EXACT) {
       am = drho * (T(0.5) + T(0.5) * vn * g.inv_cs) - dmn * T(0.5) * g.inv_cs;
       ap = drho * (T(0.5) - T(0.5) * vn * g.inv_cs) + dmn * T(0.5) * g.inv_cs;
     } else {
@@ -264,7 +269,9 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     if (wm >= T(0)) { m0 = fl0; m1 = fl1; m2 = fl2; }
     if (wp <= T(0)) { m0 = fr0; m1 = fr1; m2 = fr2; }
     const T rstar = rl + am;
-    if ((rstar < T(0)) || (rstar + ap < T(0))) {
+    // rstar + ap = rl + drho = rr; FP32 tests rr itself.
+    const bool neg2 = Fast<T>::value ? (rr < T(0)) : (rstar + ap < T(0));
+    if ((rstar < T(0)) || neg2) {
       const T sp = T(0.5) * fmx(fabs(ul) + cs, fabs(ur) + cs);
       m0 = fa0 - sp * drho;
       m1 = fa1 - sp * dmn;
