@@ -149,6 +149,25 @@ int jx_apply_drag(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* U, c
 int jx_apply_kick(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* U,
                   const void* acceleration, double dt);
 
+/* ---- Forced-run resume (no reference counterpart) -------------------------------------
+ * The loop state of a run between steps.  Resuming evolve_isothermal from the state (and
+ * the state array) another run ended with continues bit for bit as if uninterrupted: the
+ * raw force is re-rendered from OU row `update` and the first dt is derived from the state
+ * and prev_dt exactly as the uninterrupted loop derives it.  For serving pre-emption and
+ * checkpoint/restart of long forced runs. */
+typedef struct jx_loop_state {
+  double time;     /* simulation time (float64) */
+  double prev_dt;  /* dt of the last completed step; +inf before the first */
+  int cycle;       /* completed steps */
+  int update;      /* OU row of the current raw force; -1 = none */
+} jx_loop_state;
+/* evolve_isothermal from `start` (NULL = a fresh run: time 0, no step, no OU row); the
+ * loop state at the end is written to `end` (may be NULL).  tlim and nlim are the run's
+ * totals, as in the uninterrupted run. */
+int jx_evolve_isothermal_resume(jx_ctx* ctx, const jx_run_params* p, const void* U_in,
+                                void* U_out, const jx_forcing* f, const jx_loop_state* start,
+                                jx_loop_state* end);
+
 /* ---- Learned subgrid stress in the fused stage (hydro.py:111-119) -------------------
  * Before each RK stage the library writes the stage input's primitives (recover_isothermal,
  * hydro.py:54-60) to W (ny, nx, nvar) and calls fn(user, stage, W, tau); the callback must
@@ -207,6 +226,14 @@ long long jx_batch_pool_free(const jx_batch* b);
  * the GPU.  forcing may be NULL. */
 int jx_batch_add(jx_batch* b, const jx_run_params* p, const void* U0, const jx_forcing* f,
                  int rows, int* id_out);
+/* Register an instance that resumes from a loop state (see jx_loop_state): it continues bit
+ * for bit as the run it was checkpointed from. */
+int jx_batch_add_resume(jx_batch* b, const jx_run_params* p, const void* U0,
+                        const jx_forcing* f, int rows, const jx_loop_state* start, int* id_out);
+/* Copy an instance's state to U_out and return its loop state, between rounds: a mid-step
+ * instance is reported at the start of its step, a finished step as committed.  Pre-empting
+ * = checkpoint, remove, and later jx_batch_add_resume. */
+int jx_batch_checkpoint(jx_batch* b, int id, void* U_out, jx_loop_state* state_out);
 int jx_batch_remove(jx_batch* b, int id);
 /* Run at least `rounds` rounds; returns the number of still-active instances. */
 int jx_batch_run(jx_batch* b, int rounds, int* active_out);

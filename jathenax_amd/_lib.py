@@ -30,6 +30,10 @@ class Forcing(ctypes.Structure):
     ]
 
 
+class LoopState(ctypes.Structure):
+    _fields_ = [("time", c_double), ("prev_dt", c_double), ("cycle", c_int), ("update", c_int)]
+
+
 # jx_stress_fn: int (*)(void* user, int stage, const void* W, void* tau).
 STRESS_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_int, c_void_p, c_void_p)
 
@@ -51,6 +55,9 @@ SIGNATURES = {
                                          POINTER(Forcing), c_void_p, c_void_p, STRESS_FN,
                                          c_void_p, POINTER(c_double), POINTER(c_double),
                                          POINTER(c_int)]),
+    "jx_evolve_isothermal_resume": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,
+                                            POINTER(Forcing), POINTER(LoopState),
+                                            POINTER(LoopState)]),
     "jx_isothermal_timestep": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_double,
                                        c_double, c_double, POINTER(c_double)]),
     "jx_evolve_isothermal": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,
@@ -96,6 +103,9 @@ SIGNATURES = {
     "jx_batch_pool_free": (c_longlong, [c_void_p]),
     "jx_batch_add": (c_int, [c_void_p, POINTER(RunParams), c_void_p, POINTER(Forcing), c_int,
                              POINTER(c_int)]),
+    "jx_batch_add_resume": (c_int, [c_void_p, POINTER(RunParams), c_void_p, POINTER(Forcing),
+                                    c_int, POINTER(LoopState), POINTER(c_int)]),
+    "jx_batch_checkpoint": (c_int, [c_void_p, c_int, c_void_p, POINTER(LoopState)]),
     "jx_batch_remove": (c_int, [c_void_p, c_int]),
     "jx_batch_run": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "jx_batch_poll": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double), POINTER(c_int), c_int]),

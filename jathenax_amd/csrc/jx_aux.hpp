@@ -213,6 +213,15 @@ __device__ inline void begin_step(Ctrl* c, const LoopArgs& p) {
 
 static __global__ void begin_step_kernel(Ctrl* c, LoopArgs p) { begin_step(c, p); }
 
+// Resumed run: the loop state another run ended with (time, last dt, cycle, OU row).
+static __global__ void set_loop_state_kernel(Ctrl* c, double time, double prev_dt, int cycle,
+                                             int update) {
+  c->time = time;
+  c->prev_dt = prev_dt;
+  c->cycle = cycle;
+  c->update = update;
+}
+
 // Standalone step: dt (and the kick power) come from the caller.
 static __global__ void set_step_kernel(Ctrl* c, double dt, double power) {
   c->dt = dt;

@@ -947,6 +947,30 @@ __global__ void batch_init_kernel(BatchTabs<T> b, int slot) {
   b.counters[CNT_TSTEP] = 0u;
 }
 
+// A resumed instance: the loop state it was checkpointed in.
+template <typename T>
+__global__ void batch_resume_kernel(BatchTabs<T> b, int slot, double time, double prev_dt,
+                                    int cycle, int update) {
+  Ctrl& c = b.ctrl[slot].c;
+  c.time = time;
+  c.prev_dt = prev_dt;
+  c.cycle = cycle;
+  c.update = update;
+}
+
+// Raw force of one instance's current OU row over all its tiles (a resumed forced
+// instance); waves take tiles by id, no counters.
+template <typename T>
+__global__ void __launch_bounds__(kBatchBlock) batch_render_slot_kernel(BatchTabs<T> b,
+                                                                        int slot) {
+  const BatchInst<T> I = load_const(b.inst + slot);
+  const BatchCtrl& C = b.ctrl[slot];
+  const int rows = I.rows_cap;
+  const int total = I.nstrips * ((I.ny + rows - 1) / rows);
+  const int nw = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int k = wave_first(); k < total; k += nw) batch_render_tile<T>(I, C, tile_of(I, rows, k));
+}
+
 template <typename T>
 __global__ void batch_retire_kernel(BatchTabs<T> b, int slot) {
   b.ctrl[slot].live = 0;

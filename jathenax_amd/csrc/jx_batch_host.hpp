@@ -19,7 +19,8 @@ struct BatchBase {
   virtual ~BatchBase() {}
   std::string err;
   virtual int add(const jx_run_params* p, const void* U0, const jx_forcing* f, int rows,
-                  int* id) = 0;
+                  int* id, const jx_loop_state* start = nullptr) = 0;
+  virtual int checkpoint(int id, void* U_out, jx_loop_state* state) = 0;
   virtual int remove(int id) = 0;
   virtual int run(int rounds, int* active) = 0;
   virtual int poll(int* done, double* time, int* cycle, int cap) = 0;
@@ -103,8 +104,11 @@ template <typename T> class Batch : public BatchBase {
   }
 
   int add(const jx_run_params* p, const void* U0, const jx_forcing* f, int rows,
-          int* id) override {
+          int* id, const jx_loop_state* start = nullptr) override {
     if (!p || !U0 || !id) return fail("add: null argument");
+    if (start && (!std::isfinite(start->time) || start->cycle < 0 || start->update < -1 ||
+                  !(start->prev_dt > 0.0)))
+      return fail("add: bad loop state");
     if (p->dtype != (Fast<T>::value ? 1 : 0)) return fail("add: dtype does not match batch");
     if (p->nx < 1 || p->ny < 1) return fail("add: grid must be non-empty");
     if (p->nvar < 3 || p->nvar > kBatchNV) return fail("add: batched instances have nvar 3 or 4");
@@ -202,6 +206,14 @@ template <typename T> class Batch : public BatchBase {
     if (rc) return rc;
     BatchTabs<T> tb = tabs();
     hipLaunchKernelGGL(batch_init_kernel<T>, dim3(1), dim3(1), 0, stream_, tb, s);
+    if (start) {
+      const int upd = f ? start->update : -1;
+      hipLaunchKernelGGL(batch_resume_kernel<T>, dim3(1), dim3(1), 0, stream_, tb, s,
+                         start->time, start->prev_dt, start->cycle, upd);
+      if (upd >= 0)
+        hipLaunchKernelGGL(batch_render_slot_kernel<T>, dim3(grid_), dim3(kBatchBlock), 0,
+                           stream_, tb, s);
+    }
     hipLaunchKernelGGL(batch_timestep_kernel<T>, dim3(grid_), dim3(kBatchBlock), 0, stream_, tb,
                        s);
     if (hipGetLastError() != hipSuccess) return fail("add: init launch failed");
@@ -267,6 +279,26 @@ template <typename T> class Batch : public BatchBase {
     if (time) *time = h_ctrl_[id].c.time;
     if (dt) *dt = h_ctrl_[id].c.prev_dt;
     if (cycle) *cycle = h_ctrl_[id].c.cycle;
+    return JX_OK;
+  }
+
+  // State and loop state between rounds.  An instance whose final stage has run but whose
+  // step the next control round would commit is reported committed (time + dt, cycle + 1);
+  // an instance in mid-step is reported at the start of that step, whose state A still
+  // holds (the stage registers only write A in the final stage).
+  int checkpoint(int id, void* U_out, jx_loop_state* st) override {
+    if (!st) return fail("checkpoint: null state");
+    double t = 0.0, dt = 0.0;
+    int cyc = 0;
+    int rc = result(id, U_out, &t, &dt, &cyc);
+    if (rc) return rc;
+    const BatchCtrl& C = h_ctrl_[id];
+    *st = jx_loop_state{C.c.time, C.c.prev_dt, C.c.cycle, C.c.update};
+    if (C.c.pending && C.active && C.stage == h_inst_[id].nstages - 1 && !C.c.done) {
+      st->time = C.c.time + C.c.dt;
+      st->prev_dt = C.c.dt;
+      st->cycle = C.c.cycle + 1;
+    }
     return JX_OK;
   }
 

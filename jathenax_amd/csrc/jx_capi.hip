@@ -439,7 +439,8 @@ int step_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, double dt,
 
 template <typename T>
 int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx_forcing* f,
-                double* t_out, double* dt_out, int* cyc_out, const SgsHook<T>* hook = nullptr) {
+                double* t_out, double* dt_out, int* cyc_out, const SgsHook<T>* hook = nullptr,
+                const jx_loop_state* start = nullptr, jx_loop_state* end = nullptr) {
   const size_t n = (size_t)p->nx * p->ny;
   const size_t bytes = n * p->nvar * sizeof(T);
   int rc = ensure_buffers(c, bytes);
@@ -450,12 +451,18 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     if (rc) return rc;
   }
   hipLaunchKernelGGL(jx::init_ctrl_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl);
-  if (!(0.0 < p->tlim && 0 < p->nlim)) {
+  const jx_loop_state s0 = start ? *start : jx_loop_state{0.0, INFINITY, 0, -1};
+  if (start)
+    hipLaunchKernelGGL(jx::set_loop_state_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl,
+                       s0.time, s0.prev_dt, s0.cycle, forced ? s0.update : -1);
+  if (!(s0.time < p->tlim && s0.cycle < p->nlim)) {
+    // No step to take (run.py:36-38): the state passes through.
     JX_HIP(c, hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, c->stream));
     JX_HIP(c, hipStreamSynchronize(c->stream));
-    *t_out = 0.0;
-    *dt_out = INFINITY;
-    *cyc_out = 0;
+    *t_out = s0.time;
+    *dt_out = s0.prev_dt;
+    *cyc_out = s0.cycle;
+    if (end) *end = s0;
     return JX_OK;
   }
   StepPlan<T> s = make_plan<T>(c, p, hook != nullptr);
@@ -487,6 +494,14 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     ra.n_updates = f->n_updates;
     ra.ctrl = c->ctrl;
     JX_HIP(c, hipMemsetAsync(c->raw, 0, n * 2 * sizeof(T), c->stream));
+    if (s0.update >= 0) {
+      // Resume: the raw force of the OU row in effect (forcing.py:111-122).
+      jx::RenderArgs<T> r0 = ra;
+      r0.ctrl = nullptr;
+      r0.row = s0.update;
+      hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+                         c->stream, r0);
+    }
   }
   const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
   // Initial timestep from the caller's state (run.py:42-46).
@@ -517,6 +532,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
       *t_out = h.time;
       *dt_out = h.prev_dt;
       *cyc_out = h.cycle;
+      if (end) *end = jx_loop_state{h.time, h.prev_dt, h.cycle, h.update};
       return JX_OK;
     }
     // The begin_step above already opened the next step: launch it, then a chunk more.
@@ -627,6 +643,36 @@ int viscous_impl(jx_ctx* c, int ny, int nx, int nv, const T* W, double dx, doubl
                      nx, ny, nv, axis, odx, ody, o4dx, o4dy, vp);
   JX_HIP(c, hipGetLastError());
   return JX_OK;
+}
+
+// evolve_isothermal with an optional stress callback and loop state (shared checks).
+int evolve_entry(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
+                 const jx_forcing* f, void* W, void* tau, jx_stress_fn fn, void* user,
+                 double* time_out, double* dt_out, int* cycle_out, const jx_loop_state* start,
+                 jx_loop_state* end) {
+  if (!c || !time_out || !dt_out || !cycle_out) return JX_EINVAL;
+  if (fn && (!W || !tau)) return fail(c, JX_EINVAL, "a stress callback needs W and tau buffers");
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!std::isfinite(p->dedt) || p->dedt < 0.0)
+    return fail(c, JX_EINVAL, "dedt must be finite and nonnegative");
+  if (f && (!(p->dt_update > 0.0) || !std::isfinite(p->dt_update) ||
+            !std::isfinite(p->tdriv_start)))
+    return fail(c, JX_EINVAL, "dt_update must be finite and positive and tdriv_start finite");
+  if (f && (!f->coefficients || !f->cos_x || !f->sin_x || !f->cos_y || !f->sin_y ||
+            f->n_modes < 1 || f->n_updates < 1))
+    return fail(c, JX_EINVAL, "incomplete forcing tables");
+  if (!U_in || !U_out || U_in == U_out)
+    return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
+  if (p->dtype == 0) {
+    SgsHook<double> h{static_cast<double*>(W), static_cast<double*>(tau), fn, user};
+    return evolve_impl<double>(c, p, static_cast<const double*>(U_in),
+                               static_cast<double*>(U_out), f, time_out, dt_out, cycle_out,
+                               fn ? &h : nullptr, start, end);
+  }
+  SgsHook<float> h{static_cast<float*>(W), static_cast<float*>(tau), fn, user};
+  return evolve_impl<float>(c, p, static_cast<const float*>(U_in), static_cast<float*>(U_out), f,
+                            time_out, dt_out, cycle_out, fn ? &h : nullptr, start, end);
 }
 
 }  // namespace
@@ -779,32 +825,24 @@ int jx_evolve_isothermal(jx_ctx* c, const jx_run_params* p, const void* U_in, vo
                                   time_out, dt_out, cycle_out);
 }
 
+int jx_evolve_isothermal_resume(jx_ctx* c, const jx_run_params* p, const void* U_in,
+                                void* U_out, const jx_forcing* f, const jx_loop_state* start,
+                                jx_loop_state* end) {
+  if (!c) return JX_EINVAL;
+  if (start && (!std::isfinite(start->time) || start->cycle < 0 || start->update < -1 ||
+                !(start->prev_dt > 0.0)))
+    return fail(c, JX_EINVAL, "bad loop state");
+  double t, dt;
+  int cyc;
+  return evolve_entry(c, p, U_in, U_out, f, nullptr, nullptr, nullptr, nullptr, &t, &dt, &cyc,
+                      start, end);
+}
+
 int jx_evolve_isothermal_sgs(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
                              const jx_forcing* f, void* W, void* tau, jx_stress_fn fn,
                              void* user, double* time_out, double* dt_out, int* cycle_out) {
-  if (!c || !time_out || !dt_out || !cycle_out) return JX_EINVAL;
-  if (fn && (!W || !tau)) return fail(c, JX_EINVAL, "a stress callback needs W and tau buffers");
-  int rc = check_params(c, p);
-  if (rc) return rc;
-  if (!std::isfinite(p->dedt) || p->dedt < 0.0)
-    return fail(c, JX_EINVAL, "dedt must be finite and nonnegative");
-  if (f && (!(p->dt_update > 0.0) || !std::isfinite(p->dt_update) ||
-            !std::isfinite(p->tdriv_start)))
-    return fail(c, JX_EINVAL, "dt_update must be finite and positive and tdriv_start finite");
-  if (f && (!f->coefficients || !f->cos_x || !f->sin_x || !f->cos_y || !f->sin_y ||
-            f->n_modes < 1 || f->n_updates < 1))
-    return fail(c, JX_EINVAL, "incomplete forcing tables");
-  if (!U_in || !U_out || U_in == U_out)
-    return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
-  if (p->dtype == 0) {
-    SgsHook<double> h{static_cast<double*>(W), static_cast<double*>(tau), fn, user};
-    return evolve_impl<double>(c, p, static_cast<const double*>(U_in),
-                               static_cast<double*>(U_out), f, time_out, dt_out, cycle_out,
-                               fn ? &h : nullptr);
-  }
-  SgsHook<float> h{static_cast<float*>(W), static_cast<float*>(tau), fn, user};
-  return evolve_impl<float>(c, p, static_cast<const float*>(U_in), static_cast<float*>(U_out), f,
-                            time_out, dt_out, cycle_out, fn ? &h : nullptr);
+  return evolve_entry(c, p, U_in, U_out, f, W, tau, fn, user, time_out, dt_out, cycle_out,
+                      nullptr, nullptr);
 }
 
 int jx_recover_isothermal(jx_ctx* c, int dtype, long long ncells, int nvar, const void* U,
@@ -1083,6 +1121,15 @@ long long jx_batch_pool_free(const jx_batch* b) { return b ? b->impl->pool_free(
 int jx_batch_add(jx_batch* b, const jx_run_params* p, const void* U0, const jx_forcing* f,
                  int rows, int* id_out) {
   return b ? b->impl->add(p, U0, f, rows, id_out) : JX_EINVAL;
+}
+
+int jx_batch_add_resume(jx_batch* b, const jx_run_params* p, const void* U0,
+                        const jx_forcing* f, int rows, const jx_loop_state* start, int* id_out) {
+  return b ? b->impl->add(p, U0, f, rows, id_out, start) : JX_EINVAL;
+}
+
+int jx_batch_checkpoint(jx_batch* b, int id, void* U_out, jx_loop_state* state_out) {
+  return b ? b->impl->checkpoint(id, U_out, state_out) : JX_EINVAL;
 }
 
 int jx_batch_remove(jx_batch* b, int id) { return b ? b->impl->remove(id) : JX_EINVAL; }

@@ -83,8 +83,10 @@ class Ensemble:
 
     def add(self, U0, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, viscosity=0.0,
             smagorinsky_cs=0.0, drag_rate=0.0, forcing=None, dt_update=1.0, tdriv_start=0.0,
-            dedt=0.0, reconstruct="ppm4", rsolver="roe", rows=0):
-        """Register one run (evolve_isothermal's arguments); returns its instance id."""
+            dedt=0.0, reconstruct="ppm4", rsolver="roe", rows=0, state=None):
+        """Register one run (evolve_isothermal's arguments); returns its instance id.
+        `state` (a run.LoopState) resumes a checkpointed run: U0 and state as returned by
+        checkpoint() continue bit for bit."""
         for name, value in (("viscosity", viscosity), ("smagorinsky_cs", smagorinsky_cs),
                             ("drag_rate", drag_rate), ("dedt", dedt)):
             if not math.isfinite(value) or value < 0.0:
@@ -111,12 +113,31 @@ class Ensemble:
         # The batch runs on its own stream: make the caller's writes visible first.
         torch.cuda.current_stream(self.device).synchronize()
         iid = ctypes.c_int()
-        self._check(_lib.load().jx_batch_add(self.handle, ctypes.byref(p),
-                                             ctypes.c_void_p(U0.data_ptr()), fptr, int(rows),
-                                             ctypes.byref(iid)), "add")
+        if state is None:
+            self._check(_lib.load().jx_batch_add(self.handle, ctypes.byref(p),
+                                                 ctypes.c_void_p(U0.data_ptr()), fptr,
+                                                 int(rows), ctypes.byref(iid)), "add")
+        else:
+            s0 = _lib.LoopState(time=float(state.time), prev_dt=float(state.dt),
+                                cycle=int(state.cycle), update=int(state.update))
+            self._check(_lib.load().jx_batch_add_resume(self.handle, ctypes.byref(p),
+                                                        ctypes.c_void_p(U0.data_ptr()), fptr,
+                                                        int(rows), ctypes.byref(s0),
+                                                        ctypes.byref(iid)), "add")
         self._keep[iid.value] = keep[1:]   # forcing tables must outlive the instance
         self._shapes[iid.value] = tuple(U0.shape)
         return iid.value
+
+    def checkpoint(self, iid):
+        """(U copy, run.LoopState) of an instance between rounds, for pre-emption:
+        remove() it, and later add(U, ..., state=...) continues it bit for bit."""
+        from .run import LoopState
+        U = torch.empty(self._shapes[iid], dtype=self.dtype, device=self.device)
+        st = _lib.LoopState()
+        self._check(_lib.load().jx_batch_checkpoint(self.handle, int(iid),
+                                                    ctypes.c_void_p(U.data_ptr()),
+                                                    ctypes.byref(st)), "checkpoint")
+        return U, LoopState(st.time, st.prev_dt, st.cycle, st.update)
 
     def remove(self, iid):
         self._check(_lib.load().jx_batch_remove(self.handle, int(iid)), "remove")

@@ -10,6 +10,7 @@ tuple of forcing.forcing_basis.
 
 import ctypes
 import math
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -42,10 +43,39 @@ def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
                    predict_stress=predict_stress, sgs_weights=sgs_weights)
 
 
+@dataclass(frozen=True)
+class LoopState:
+    """Where a run stands between steps: simulation time (float64), the last step's dt
+    (+inf before the first), completed steps, and the OU row of the raw force (-1: none)."""
+    time: float = 0.0
+    dt: float = math.inf
+    cycle: int = 0
+    update: int = -1
+
+
+def evolve_resumable(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, state=None,
+                     viscosity=0.0, smagorinsky_cs=0.0, drag_rate=0.0, forcing=None,
+                     dt_update=1.0, tdriv_start=0.0, dedt=0.0, reconstruct="ppm4",
+                     rsolver="roe"):
+    """evolve_isothermal that starts from `state` (a LoopState; None = a fresh run) and
+    returns (U, LoopState).  Resuming from the state and array a run ended with continues
+    bit for bit as the uninterrupted run: checkpoint/restart of long forced runs and
+    pre-emption in serving (no reference counterpart)."""
+    for name, value in (("viscosity", viscosity), ("smagorinsky_cs", smagorinsky_cs),
+                        ("drag_rate", drag_rate), ("dedt", dedt)):
+        if not math.isfinite(value) or value < 0.0:
+            raise ValueError(f"{name} must be finite and nonnegative")
+    return _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, viscosity=viscosity,
+                   smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
+                   dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt,
+                   reconstruct=reconstruct, rsolver=rsolver, apply_table=True,
+                   start=state or LoopState(), want_state=True)
+
+
 def _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, viscosity=0.0,
             smagorinsky_cs=0.0, drag_rate=0.0, forcing=None, dt_update=1.0, tdriv_start=0.0,
             dedt=0.0, reconstruct="ppm4", rsolver="roe", apply_table=True,
-            predict_stress=None, sgs_weights=None):
+            predict_stress=None, sgs_weights=None, start=None, want_state=False):
     U = as_device(U)
     p = run_params(U, dx, dy, cs, dfloor, integrator, reconstruct=reconstruct, rsolver=rsolver,
                    viscosity=viscosity, smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate,
@@ -72,6 +102,15 @@ def _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *, viscosity=0.0
     ctx = context_for(U)
     if apply_table:
         tuning.apply_single(ctx, p)
+    if want_state:
+        s0 = _lib.LoopState(time=float(start.time), prev_dt=float(start.dt),
+                            cycle=int(start.cycle), update=int(start.update))
+        s1 = _lib.LoopState()
+        ctx.check(_lib.load().jx_evolve_isothermal_resume(ctx.handle, ctypes.byref(p), ptr(U),
+                                                          ptr(out), fptr, ctypes.byref(s0),
+                                                          ctypes.byref(s1)), "evolve_isothermal")
+        del keep
+        return out, LoopState(s1.time, s1.prev_dt, s1.cycle, s1.update)
     if predict_stress is None:
         ctx.check(_lib.load().jx_evolve_isothermal(ctx.handle, ctypes.byref(p), ptr(U), ptr(out),
                                                    fptr, ctypes.byref(t), ctypes.byref(dt),

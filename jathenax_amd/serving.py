@@ -5,8 +5,14 @@ request's state resident in one Ensemble (a single HBM pool) and advances all of
 together, one RK stage per round, so requests that arrive while others are running join
 at the next round boundary and their work overlaps on the device.  Admission is by HBM
 footprint: a request is admitted when its registers fit the pool's free space and an
-instance slot is free; otherwise it waits in FIFO order.  Finished requests are retired
-between rounds, which frees their memory for the queue.
+instance slot is free; otherwise it waits.  Arrived requests are admitted in priority order
+(highest first), then in arrival order.  Finished requests are retired between rounds,
+which frees their memory for the queue.
+
+Pre-emption: when the next request cannot be admitted and lower-priority requests are
+running, the lowest-priority running request is checkpointed (state and loop state),
+removed and requeued; resumed later, it continues bit for bit, so pre-emption changes only
+latency.
 
 Failure handling: a request whose time or state is non-finite, or that exceeds its round
 budget, is evicted with an error status instead of blocking the batch.
@@ -39,6 +45,9 @@ class Request:
     result: object = None          # (U on device, time, dt, cycle)
     iid: int = -1
     rounds: int = 0
+    priority: int = 0              # larger is more urgent
+    checkpoint: object = None      # (U on device, run.LoopState) while pre-empted
+    preemptions: int = 0
     info: dict = field(default_factory=dict)
 
     @property
@@ -48,12 +57,13 @@ class Request:
 
 class Server:
     def __init__(self, pool_bytes=None, max_instances=256, dtype=torch.float32,
-                 rounds_per_poll=6, keep_results=True, graph=True):
+                 rounds_per_poll=6, keep_results=True, graph=True, preempt=True):
         self.ens = Ensemble(dtype=dtype, max_instances=max_instances, pool_bytes=pool_bytes,
                             graph=graph, graph_rounds=rounds_per_poll)
         self.dtype = dtype
         self.rounds_per_poll = rounds_per_poll
         self.keep_results = keep_results
+        self.preempt = preempt
         self.queue = deque()
         self.running = {}
         self.finished = []
@@ -63,33 +73,70 @@ class Server:
         return time.perf_counter() - self.clock0
 
     def submit(self, req):
-        self.queue.append(req)
+        """Queue a request: by priority (highest first), then arrival."""
+        items = list(self.queue) + [req]
+        items.sort(key=lambda r: (-r.priority, r.arrival))
+        self.queue = deque(items)
+
+    def _fits(self, req):
+        return (len(self.running) < self.ens.max_instances
+                and self._footprint(req.spec) <= self.ens.pool_free())
+
+    def _preempt_for(self, req):
+        """Checkpoint and requeue lower-priority running requests until `req` fits."""
+        victims = sorted((r for r in self.running.values() if r.priority < req.priority),
+                         key=lambda r: (r.priority, -r.admitted))
+        for v in victims:
+            if self._fits(req):
+                break
+            v.checkpoint = self.ens.checkpoint(v.iid)
+            self.ens.remove(v.iid)
+            del self.running[v.iid]
+            v.iid, v.status = -1, "queued"
+            v.preemptions += 1
+            self.submit(v)
+        return self._fits(req)
 
     def _footprint(self, spec):
         ny, nx, nv = spec["U"].shape
         return self.ens.footprint(nx, ny, nv, forced="forcing" in spec["kw"]
                                   and spec["kw"]["forcing"] is not None)
 
+    def _next(self):
+        """The highest-priority request that has arrived (the queue is sorted)."""
+        now = self.now()
+        return next((r for r in self.queue if r.arrival <= now), None)
+
     def _admit(self):
-        while self.queue and self.queue[0].arrival <= self.now():
-            req = self.queue[0]
-            if len(self.running) >= self.ens.max_instances:
+        while True:
+            req = self._next()
+            if req is None:
                 return
-            if self._footprint(req.spec) > self.ens.pool_free():
-                if not self.running:
+            if not self._fits(req):
+                if self.preempt and self._preempt_for(req):
+                    pass
+                elif not self.running:
                     # Cannot ever fit: reject rather than stall the queue.
-                    self.queue.popleft()
+                    self.queue.remove(req)
                     req.status, req.finished = "error", self.now()
                     req.info["error"] = "request exceeds the HBM pool"
                     self.finished.append(req)
                     continue
-                return
-            self.queue.popleft()
+                else:
+                    return
+            self.queue.remove(req)
             s = req.spec
-            req.iid = self.ens.add(torch.as_tensor(s["U"], dtype=self.dtype, device="cuda"),
-                                   s["dx"], s["dy"], s["cs"], s["dfloor"], s["integrator"],
-                                   s["cfl"], s["tlim"], s["nlim"], rows=req.rows, **s["kw"])
-            req.admitted, req.status = self.now(), "running"
+            if req.checkpoint is not None:
+                U0, state = req.checkpoint
+                req.checkpoint = None
+            else:
+                U0, state = torch.as_tensor(s["U"], dtype=self.dtype, device="cuda"), None
+            req.iid = self.ens.add(U0, s["dx"], s["dy"], s["cs"], s["dfloor"], s["integrator"],
+                                   s["cfl"], s["tlim"], s["nlim"], rows=req.rows, state=state,
+                                   **s["kw"])
+            if math.isnan(req.admitted):
+                req.admitted = self.now()
+            req.status = "running"
             self.running[req.iid] = req
 
     def _retire(self):
@@ -115,7 +162,7 @@ class Server:
 
     def serve(self, requests, timeout=None):
         """Run until every request has finished (or `timeout` seconds elapsed)."""
-        for r in sorted(requests, key=lambda r: r.arrival):
+        for r in requests:
             self.submit(r)
         self.clock0 = time.perf_counter()
         while self.queue or self.running:
@@ -124,7 +171,7 @@ class Server:
                 self.ens.run(self.rounds_per_poll)
                 self._retire()
             elif self.queue:
-                wait = self.queue[0].arrival - self.now()
+                wait = min(r.arrival for r in self.queue) - self.now()
                 if wait > 0:
                     time.sleep(min(wait, 0.01))
             if timeout is not None and self.now() > timeout:
