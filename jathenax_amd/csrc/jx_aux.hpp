@@ -243,35 +243,103 @@ template <typename T> struct RenderArgs {
   int row;
 };
 
+// The raw force of one wave's 64 columns over rows y0 .. y0+nr-1 (nr <= 8).  Modes go in
+// chunks of up to kRenderModes: the wave stages the chunk's row tables (cy, sy of its rows)
+// in its LDS slice `lds` (2 * kRenderModes * 8 values), then each lane loads its column's
+// cx, sx once per mode and accumulates the 8 rows in registers.  Each cell sums its modes in
+// order with the reference's expression (forcing.py:115-122), so the result is the same bit
+// for bit as a per-cell loop.  Lanes with on == false follow the uniform control flow.
+constexpr int kRenderModes = 32;
+
+__device__ __forceinline__ void render_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, typename PR>
+__device__ __forceinline__ void render_wave(PR raw, const T* amp_, bool fill, const T* cx_,
+                                            const T* sx_, const T* cy_, const T* sy_, int nx,
+                                            int ny, int n_modes, int i, bool on, int y0, int nr,
+                                            T* lds) {
+  const gmem_t<T>* cx = gmem(cx_);
+  const gmem_t<T>* sx = gmem(sx_);
+  const gmem_t<T>* cy = gmem(cy_);
+  const gmem_t<T>* sy = gmem(sy_);
+  const cmem_t<T>* amp = (const cmem_t<T>*)amp_;   // uniform, read-only: scalar loads
+  const int lane = threadIdx.x & 63;
+  const T nan = T(__builtin_nan(""));
+  T f0[8], f1[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) f0[r] = f1[r] = T(0);
+  for (int m0 = 0; m0 < n_modes; m0 += kRenderModes) {
+    const int mc = min(kRenderModes, n_modes - m0);
+    render_wave_sync();   // the previous chunk's reads are done
+    for (int e = lane; e < mc * 8; e += 64) {
+      const int q = e >> 3, r = e & 7;
+      const size_t at = (size_t)(m0 + q) * ny + (y0 + (r < nr ? r : 0));
+      lds[2 * e + 0] = cy[at];
+      lds[2 * e + 1] = sy[at];
+    }
+    render_wave_sync();
+    for (int q = 0; q < mc; ++q) {
+      const int m = m0 + q;
+      const T cxv = on ? cx[(size_t)m * nx + i] : T(0);
+      const T sxv = on ? sx[(size_t)m * nx + i] : T(0);
+      const cmem_t<T>* am = amp + (size_t)m * 4;   // [dir][re/im]
+      const T a00 = fill ? nan : am[0], a01 = fill ? nan : am[1];
+      const T a10 = fill ? nan : am[2], a11 = fill ? nan : am[3];
+      const T* ly = lds + 16 * q;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        if (r < nr) {
+          const T cyv = ly[2 * r], syv = ly[2 * r + 1];
+          const T re = cxv * cyv - sxv * syv;
+          const T im = cxv * syv + sxv * cyv;
+          f0[r] = f0[r] + (re * a00 - im * a01);
+          f1[r] = f1[r] + (re * a10 - im * a11);
+        }
+      }
+    }
+  }
+  if (on) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r < nr) {
+        raw[((size_t)(y0 + r) * nx + i) * 2 + 0] = f0[r];
+        raw[((size_t)(y0 + r) * nx + i) * 2 + 1] = f1[r];
+      }
+    }
+  }
+}
+
+// Rows of one render block; blocks are kAuxBlock columns wide (four waves, each with its own
+// LDS slice) and kRenderRows rows tall, or 8 rows when the grid is small.
+constexpr int kRenderRows = 16;
+
+__host__ __device__ inline int render_rows_per_block(int nx, int ny) {
+  const long long blocks = (long long)((nx + 255) / 256) * ((ny + 15) / 16);
+  return blocks >= 1024 ? kRenderRows : 8;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kAuxBlock) render_force_kernel(RenderArgs<T> a) {
+  __shared__ T slices[kAuxBlock / 64][2 * kRenderModes * 8];
   int row = a.row;
   if (a.ctrl) {
     if (a.ctrl->done || !a.ctrl->render) return;
     row = a.ctrl->update;
   }
-  const size_t n = (size_t)a.nx * a.ny;
   const bool fill = (row < 0) || (row >= a.n_updates);
   const T* amp = a.table + (size_t)(fill ? 0 : row) * a.n_modes * 4;
-  const T nan = T(__builtin_nan(""));
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int j = (int)(c / a.nx), i = (int)(c % a.nx);
-    T f0 = T(0), f1 = T(0);
-    for (int m = 0; m < a.n_modes; ++m) {
-      const T cxv = a.cx[(size_t)m * a.nx + i], sxv = a.sx[(size_t)m * a.nx + i];
-      const T cyv = a.cy[(size_t)m * a.ny + j], syv = a.sy[(size_t)m * a.ny + j];
-      const T re = cxv * cyv - sxv * syv;
-      const T im = cxv * syv + sxv * cyv;
-      const T* am = amp + (size_t)m * 4;   // [dir][re/im]
-      const T a00 = fill ? nan : am[0], a01 = fill ? nan : am[1];
-      const T a10 = fill ? nan : am[2], a11 = fill ? nan : am[3];
-      f0 = f0 + (re * a00 - im * a01);
-      f1 = f1 + (re * a10 - im * a11);
-    }
-    a.raw[c * 2 + 0] = f0;
-    a.raw[c * 2 + 1] = f1;
-  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = i < a.nx;
+  const int rb = render_rows_per_block(a.nx, a.ny);
+  const int y0 = blockIdx.y * rb;
+  T* lds = slices[threadIdx.x >> 6];
+  for (int yb = y0; yb < min(y0 + rb, a.ny); yb += 8)
+    render_wave<T>(gmem(a.raw), amp, fill, a.cx, a.sx, a.cy, a.sy, a.nx, a.ny, a.n_modes,
+                   on ? i : 0, on, yb, min(8, a.ny - yb), lds);
 }
 
 template <typename T> struct KickArgs {
@@ -508,7 +576,8 @@ __global__ void __launch_bounds__(kAuxBlock) box_sums_kernel(const T* u, size_t 
     for (int v = 0; v < nv; ++v) partial[(size_t)blockIdx.x * 8 + v] = s[v][0];
 }
 
-static __global__ void box_sums_finish_kernel(const double* partial, int nblocks, int nv, double* out) {
+static __global__ void box_sums_finish_kernel(const double* partial, int nblocks, int nv,
+                                              double* out) {
   if (threadIdx.x != 0) return;
   for (int v = 0; v < nv; ++v) {
     double a = 0.0;

@@ -89,33 +89,6 @@ template <typename T> struct BatchTabs {
   int fill;               // tiles per wave the band heights aim for
 };
 
-// Pointers read from the instance table are generic (flat) to the compiler.  Flat accesses
-// also count against the LDS wait counter, so every LDS wait in the row march would drain
-// the row prefetch; casting them to the global address space keeps them on the vector
-// memory path.
-template <typename T> using gmem_t = T __attribute__((address_space(1)));
-template <typename T> __device__ __forceinline__ gmem_t<T>* gmem(T* p) {
-  return (gmem_t<T>*)p;
-}
-template <typename T> __device__ __forceinline__ const gmem_t<T>* gmem(const T* p) {
-  return (const gmem_t<T>*)p;
-}
-
-// The instance table (written by the host between rounds) and the work lists (written by
-// the control kernel, an earlier launch) are read-only while a worker kernel runs, so they
-// are read through the constant address space: uniform addresses then become scalar loads
-// into SGPRs instead of vector loads into VGPRs.
-template <typename T> using cmem_t = T __attribute__((address_space(4)));
-template <typename S> __device__ __forceinline__ S load_const(const S* p) {
-  static_assert(sizeof(S) % 4 == 0, "record must be a whole number of dwords");
-  S out;
-  const cmem_t<int>* src = (const cmem_t<int>*)p;
-  int* dst = reinterpret_cast<int*>(&out);
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(S) / 4); ++i) dst[i] = src[i];
-  return out;
-}
-
 // k-th tile of work list L (k < total): instance slot s and its tile q.
 template <typename T>
 __device__ __forceinline__ void list_item(const BatchTabs<T>& b, const int* L, int k, int& s,
@@ -565,7 +538,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
 
 template <typename T>
 __device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C,
-                                                  const Tile& tl);
+                                                  const Tile& tl, T* lds);
 
 template <typename T, int NV>
 __device__ __forceinline__ void batch_stage_dispatch(const BatchInst<T>& I, BatchCtrl& C,
@@ -597,7 +570,7 @@ __global__ void __launch_bounds__(kBatchBlock, 2) batch_stage_kernel(BatchTabs<T
     else batch_stage_dispatch<T, 3>(I, C, tl, lds);
     // A step that opened a new forcing row renders the raw force of this tile (read by
     // the kick at the step's final stage).
-    if (C.stage == 0 && I.forced && C.c.render) batch_render_tile<T>(I, C, tl);
+    if (C.stage == 0 && I.forced && C.c.render) batch_render_tile<T>(I, C, tl, lds);
   }
 }
 
@@ -707,33 +680,16 @@ __global__ void __launch_bounds__(kCtrlBlock) batch_control_kernel(BatchTabs<T> 
 
 template <typename T>
 __device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const BatchCtrl& C,
-                                                  const Tile& tl) {
+                                                  const Tile& tl, T* lds) {
   const int t = threadIdx.x & 63;
   const int row = C.c.update;
   const bool fill = (row < 0) || (row >= I.n_updates);
-  const gmem_t<T>* amp = gmem(I.coef) + (size_t)(fill ? 0 : row) * I.n_modes * 4;
-  const gmem_t<T>*cx = gmem(I.cx), *sx = gmem(I.sx), *cy = gmem(I.cy), *sy = gmem(I.sy);
-  gmem_t<T>* raw = gmem(I.raw);
-  const T nan = T(__builtin_nan(""));
+  const T* amp = I.coef + (size_t)(fill ? 0 : row) * I.n_modes * 4;
   const int i = tl.x0 + t;
-  const bool on = t < kTileCols && i < I.nx;   // per lane; the tile loop stays uniform
-  const int yend = on ? min(tl.y0 + tl.rows, I.ny) : tl.y0;
-  for (int j = tl.y0; j < yend; ++j) {
-    T f0 = T(0), f1 = T(0);
-    for (int m = 0; m < I.n_modes; ++m) {
-      const T cxv = cx[(size_t)m * I.nx + i], sxv = sx[(size_t)m * I.nx + i];
-      const T cyv = cy[(size_t)m * I.ny + j], syv = sy[(size_t)m * I.ny + j];
-      const T re = cxv * cyv - sxv * syv;
-      const T im = cxv * syv + sxv * cyv;
-      const gmem_t<T>* am = amp + (size_t)m * 4;
-      const T a00 = fill ? nan : am[0], a01 = fill ? nan : am[1];
-      const T a10 = fill ? nan : am[2], a11 = fill ? nan : am[3];
-      f0 = f0 + (re * a00 - im * a01);
-      f1 = f1 + (re * a10 - im * a11);
-    }
-    raw[((size_t)j * I.nx + i) * 2 + 0] = f0;
-    raw[((size_t)j * I.nx + i) * 2 + 1] = f1;
-  }
+  const bool on = t < kTileCols && i < I.nx;   // per lane; the row loop stays uniform
+  for (int yb = tl.y0; yb < tl.y0 + tl.rows; yb += 8)
+    render_wave<T>(gmem(I.raw), amp, fill, I.cx, I.sx, I.cy, I.sy, I.nx, I.ny, I.n_modes,
+                   on ? i : 0, on, yb, min(8, tl.y0 + tl.rows - yb), lds);
 }
 
 __device__ __forceinline__ bool kicking(const BatchCtrl& C, int forced, int nstages) {
@@ -963,12 +919,15 @@ __global__ void batch_resume_kernel(BatchTabs<T> b, int slot, double time, doubl
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_render_slot_kernel(BatchTabs<T> b,
                                                                         int slot) {
+  __shared__ T slices[kBatchBlock / 64][2 * kRenderModes * 8];
   const BatchInst<T> I = load_const(b.inst + slot);
   const BatchCtrl& C = b.ctrl[slot];
   const int rows = I.rows_cap;
   const int total = I.nstrips * ((I.ny + rows - 1) / rows);
   const int nw = (int)(gridDim.x * (blockDim.x >> 6));
-  for (int k = wave_first(); k < total; k += nw) batch_render_tile<T>(I, C, tile_of(I, rows, k));
+  T* lds = slices[threadIdx.x >> 6];
+  for (int k = wave_first(); k < total; k += nw)
+    batch_render_tile<T>(I, C, tile_of(I, rows, k), lds);
 }
 
 template <typename T>

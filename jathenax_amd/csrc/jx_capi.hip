@@ -115,6 +115,11 @@ int ensure_buffers(jx_ctx* c, size_t bytes) {
   return JX_OK;
 }
 
+dim3 render_grid(int nx, int ny) {
+  const int rb = jx::render_rows_per_block(nx, ny);
+  return dim3((unsigned)((nx + jx::kAuxBlock - 1) / jx::kAuxBlock), (unsigned)((ny + rb - 1) / rb));
+}
+
 dim3 aux_grid(size_t n) {
   size_t b = (n + jx::kAuxBlock - 1) / jx::kAuxBlock;
   b = std::max<size_t>(1, std::min<size_t>(b, 4096));
@@ -499,8 +504,8 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
       jx::RenderArgs<T> r0 = ra;
       r0.ctrl = nullptr;
       r0.row = s0.update;
-      hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
-                         c->stream, r0);
+      hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(p->nx, p->ny),
+                         dim3(jx::kAuxBlock), 0, c->stream, r0);
     }
   }
   const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
@@ -510,8 +515,8 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
   auto one_step = [&](bool first) -> int {
     hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
     if (forced)
-      hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
-                         c->stream, ra);
+      hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(p->nx, p->ny),
+                         dim3(jx::kAuxBlock), 0, c->stream, ra);
     int r = launch_step<T>(c, s, in, out, first, raw, !smag, hook);
     if (r) return r;
     if (smag)
@@ -545,8 +550,8 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     chunk = hook ? 1 : std::max<long long>(1, std::min<long long>(want, 512));
     {
       if (forced)
-        hipLaunchKernelGGL(jx::render_force_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
-                           c->stream, ra);
+        hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(p->nx, p->ny),
+                           dim3(jx::kAuxBlock), 0, c->stream, ra);
       int r = launch_step<T>(c, s, in, out, false, raw, !smag, hook);
       if (r) return r;
       if (smag)
@@ -926,7 +931,7 @@ int jx_render_force(jx_ctx* c, int dtype, int ny, int nx, int n_modes, const voi
   if (!c) return JX_EINVAL;
   if (ny < 1 || nx < 1 || n_modes < 0 || !raw_out || (n_modes > 0 && !amplitudes))
     return fail(c, JX_EINVAL, "bad render_force arguments");
-  const dim3 g = aux_grid((size_t)nx * ny);
+  const dim3 g = render_grid(nx, ny);
   if (dtype == 0) {
     jx::RenderArgs<double> a{static_cast<double*>(raw_out), static_cast<const double*>(amplitudes),
                              static_cast<const double*>(cos_x), static_cast<const double*>(sin_x),

@@ -15,6 +15,33 @@
 
 namespace jx {
 
+// Pointers read from the instance table are generic (flat) to the compiler.  Flat accesses
+// also count against the LDS wait counter, so every LDS wait in the row march would drain
+// the row prefetch; casting them to the global address space keeps them on the vector
+// memory path.
+template <typename T> using gmem_t = T __attribute__((address_space(1)));
+template <typename T> __device__ __forceinline__ gmem_t<T>* gmem(T* p) {
+  return (gmem_t<T>*)p;
+}
+template <typename T> __device__ __forceinline__ const gmem_t<T>* gmem(const T* p) {
+  return (const gmem_t<T>*)p;
+}
+
+// The instance table (written by the host between rounds) and the work lists (written by
+// the control kernel, an earlier launch) are read-only while a worker kernel runs, so they
+// are read through the constant address space: uniform addresses then become scalar loads
+// into SGPRs instead of vector loads into VGPRs.
+template <typename T> using cmem_t = T __attribute__((address_space(4)));
+template <typename S> __device__ __forceinline__ S load_const(const S* p) {
+  static_assert(sizeof(S) % 4 == 0, "record must be a whole number of dwords");
+  S out;
+  const cmem_t<int>* src = (const cmem_t<int>*)p;
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(S) / 4); ++i) dst[i] = src[i];
+  return out;
+}
+
 template <typename T> struct Fast { static constexpr bool value = false; };
 template <> struct Fast<float> { static constexpr bool value = true; };
 
