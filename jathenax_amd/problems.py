@@ -14,7 +14,8 @@ from . import forcing as _forcing
 
 
 def cell_centres(n):
-    return (np.arange(n) + 0.5) / n - 0.5
+    """x_i = (i + 1/2) dx - 1/2 with dx = 1/n, evaluated as the CLI does (run.py:133-135)."""
+    return (np.arange(n) + 0.5) * (1.0 / n) - 0.5
 
 
 def smooth_reference(nx, ny):
