@@ -53,16 +53,14 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
   __shared__ unsigned long long sred[2 * (kAuxBlock / 64)];
   const int nx = a.nx, ny = a.ny, nv = a.nv;
   const size_t n = (size_t)nx * ny;
-  unsigned long long kmin = ~0ull, kmax = 0ull;
+  CflMin<T> cfl(a.dx, a.dy, a.cs);
+  unsigned long long kmax = 0ull;
   for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (size_t)gridDim.x * blockDim.x) {
     const int j = (int)(c / nx), i = (int)(c % nx);
     T r, vx, vy;
     load_prim(a.q + c * nv, nv, a.primitive != 0, a.dfloor, r, vx, vy);
-    const unsigned long long kx = key_of((double)dvd(a.dx, fabs(vx) + a.cs));
-    const unsigned long long ky = key_of((double)dvd(a.dy, fabs(vy) + a.cs));
-    kmin = kx < kmin ? kx : kmin;
-    kmin = ky < kmin ? ky : kmin;
+    cfl.add(vx, vy);
     if (a.smag_on) {
       const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
       const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
@@ -100,6 +98,7 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
       }
     }
   }
+  unsigned long long kmin = cfl.key();
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long o = __shfl_xor(kmin, off);
     kmin = o < kmin ? o : kmin;
@@ -440,7 +439,7 @@ __global__ void __launch_bounds__(kAuxBlock) kick_apply_kernel(KickArgs<T> a) {
   const bool off = a.ctrl->power == 0.0;
   const T scale = T(a.ctrl->scale);
   const size_t n = (size_t)a.nx * a.ny;
-  unsigned long long kmin = ~0ull;
+  CflMin<T> cfl(a.dx, a.dy, a.cs);
   for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (size_t)gridDim.x * blockDim.x) {
     T* q = a.u + c * a.nv;
@@ -456,13 +455,10 @@ __global__ void __launch_bounds__(kAuxBlock) kick_apply_kernel(KickArgs<T> a) {
     q[2] = m1;
     if (a.dtkey) {
       const T inv = dvd(T(1), rf);
-      const unsigned long long kx = key_of((double)dvd(a.dx, fabs(inv * m0) + a.cs));
-      const unsigned long long ky = key_of((double)dvd(a.dy, fabs(inv * m1) + a.cs));
-      kmin = kx < kmin ? kx : kmin;
-      kmin = ky < kmin ? ky : kmin;
+      cfl.add(inv * m0, inv * m1);
     }
   }
-  if (a.dtkey) block_min_key(kmin, sred, a.dtkey);
+  if (a.dtkey) block_min_key(cfl.key(), sred, a.dtkey);
 }
 
 // Standalone normalize_force output: the acceleration field (forcing.py:149).

@@ -282,7 +282,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
       pf[v] = src[v];
   }
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-  unsigned long long kmin = ~0ull;
+  CflMin<T> cfl(I.dx, I.dy, I.cs);
 
   auto body = [&](auto pc, int i) {
     constexpr int P = decltype(pc)::value;
@@ -396,20 +396,20 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           const T* row = sW + v * 64;
-          xl[v] = ppm4_interface(row[tm2], row[tm1], row[t], row[tp1]);
+          xl[v] = ppm4_interface(row[tm2], row[tm1], w[S3][v], row[tp1]);
           sI[v * 64 + t] = xl[v];
         }
         wave_sync();
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           xh[v] = sI[v * 64 + tp1];
-          ppm4_limit(sW[v * 64 + t], xl[v], xh[v]);
+          ppm4_limit(w[S3][v], xl[v], xh[v]);
         }
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           const T* row = sW + v * 64;
-          cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], I.eps, xl[v],
+          cell_edges<RECON>(row[tm2], row[tm1], w[S3][v], row[tp1], row[tp2], I.eps, xl[v],
                             xh[v]);
         }
       }
@@ -488,11 +488,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
           un[0] = (un[0] < dfloor) ? dfloor : un[0];
           if (want_key) {
             const T inv = dvd(T(1), un[0]);
-            const T lx = dvd<T>(I.dx, T(fabs(inv * un[1])) + I.cs);
-            const T ly = dvd<T>(I.dy, T(fabs(inv * un[2])) + I.cs);
-            const unsigned long long kx = key_of((double)lx), ky = key_of((double)ly);
-            kmin = kx < kmin ? kx : kmin;
-            kmin = ky < kmin ? ky : kmin;
+            cfl.add(inv * un[1], inv * un[2]);
           }
         }
 #pragma unroll
@@ -528,6 +524,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     }
   }
   if (want_key) {
+    unsigned long long kmin = cfl.key();
     for (int off = 32; off > 0; off >>= 1) {
       const unsigned long long o = __shfl_xor(kmin, off);
       kmin = o < kmin ? o : kmin;
@@ -744,7 +741,7 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
   const bool on = t < kTileCols && i < I.nx;
   const int yend = tl.y0 + tl.rows;
   double s0 = 0, s1 = 0, s2 = 0;
-  unsigned long long kmin = ~0ull;
+  CflMin<T> cfl(I.dx, I.dy, I.cs);
   gmem_t<T>* A = gmem(I.A);
   const gmem_t<T>* raw = gmem(static_cast<const T*>(I.raw));
   gmem_t<double>* part = gmem(I.part);
@@ -782,14 +779,12 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
       q[2] = m1;
       if (!I.smag_on) {
         const T inv = dvd(T(1), rf);
-        const unsigned long long kx = key_of((double)dvd<T>(I.dx, T(fabs(inv * m0)) + I.cs));
-        const unsigned long long ky = key_of((double)dvd<T>(I.dy, T(fabs(inv * m1)) + I.cs));
-        kmin = kx < kmin ? kx : kmin;
-        kmin = ky < kmin ? ky : kmin;
+        cfl.add(inv * m0, inv * m1);
       }
     }
   }
   if (mode == 1 && !I.smag_on) {
+    unsigned long long kmin = cfl.key();
     for (int off2 = 32; off2 > 0; off2 >>= 1) {
       const unsigned long long o = __shfl_xor(kmin, off2);
       kmin = o < kmin ? o : kmin;
@@ -819,16 +814,14 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
   const int i = tl.x0 + t;
   const bool on = t < kTileCols && i < I.nx;
   const int nx = I.nx, ny = I.ny, nv = I.nv;
-  unsigned long long kmin = ~0ull, kmax = 0ull;
+  CflMin<T> cfl(I.dx, I.dy, I.cs);
+  unsigned long long kmax = 0ull;
   const gmem_t<T>* A = gmem(static_cast<const T*>(I.A));
   const int yend = on ? min(tl.y0 + tl.rows, ny) : tl.y0;
   for (int j = tl.y0; j < yend; ++j) {
     T r, vx, vy;
     load_prim(A + ((size_t)j * nx + i) * nv, nv, false, I.dfloor, r, vx, vy);
-    const unsigned long long kx = key_of((double)dvd(I.dx, fabs(vx) + I.cs));
-    const unsigned long long ky = key_of((double)dvd(I.dy, fabs(vy) + I.cs));
-    kmin = kx < kmin ? kx : kmin;
-    kmin = ky < kmin ? ky : kmin;
+    cfl.add(vx, vy);
     if (I.smag_on) {
       const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
       const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
@@ -862,6 +855,7 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
       }
     }
   }
+  unsigned long long kmin = cfl.key();
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long o = __shfl_xor(kmin, off);
     kmin = o < kmin ? o : kmin;

@@ -88,9 +88,19 @@ __device__ __forceinline__ T ppm4_interface(T qm2, T qm1, T q, T qp1) {
 }
 
 // PPM4 extremum and asymmetric-parabola limiter on clamped edges (fluxes.py:34-41).
+// FP32 uses the same limiter as two medians: with dl = lo - q and dh = hi - q,
+// dl' = med3(dl, -2 dh, 0). At an extremum (dl, dh of one sign, or one zero) dl and -2 dh
+// straddle zero and the median is 0 (lo = q). Otherwise both have the sign of dl and the
+// median is the smaller in magnitude: -2 dh exactly when |dl| >= 2 |dh|. Likewise for dh'.
+// Values agree with the select form up to one rounding of q + (lo - q).
 template <typename T>
 __device__ __forceinline__ void ppm4_limit(T q, T& lo, T& hi) {
   const T dh = hi - q, dl = lo - q;
+  if constexpr (Fast<T>::value) {
+    lo = q + __builtin_amdgcn_fmed3f(dl, T(-2) * dh, T(0));
+    hi = q + __builtin_amdgcn_fmed3f(dh, T(-2) * dl, T(0));
+    return;
+  }
   const bool ext = dh * dl >= T(0);
   const T l2 = (fabs(dl) >= T(2) * fabs(dh)) ? q - T(2) * dh : lo;
   const T h2 = (fabs(dh) >= T(2) * fabs(dl)) ? q - T(2) * dl : hi;
@@ -293,6 +303,26 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     T m0 = (fa0 + cm) + cp;
     T m1 = (fa1 + cm * wm) + cp * wp;
     T m2 = ((fa2 + cm * vt) + csh) + cp * vt;
+    if constexpr (Fast<T>::value) {
+      // Supersonic faces: with every wave speed of one sign, |lambda| = +-lambda and the
+      // Roe property A(Ur - Ul) = Fr - Fl makes the sum above equal fl (or fr) up to
+      // rounding, so FP32 keeps the sum. The LLF fallback is rare: one wave-wide vote
+      // skips it.
+      const T rstar = rl + am;
+      const bool bad = (rstar < T(0)) || (rr < T(0));
+      if (__builtin_amdgcn_ballot_w64(bad) != 0) {
+        if (bad) {
+          const T sp = T(0.5) * fmx(fabs(ul) + cs, fabs(ur) + cs);
+          m0 = fa0 - sp * drho;
+          m1 = fa1 - sp * dmn;
+          m2 = fa2 - sp * dmt;
+        }
+      }
+      f0 = m0;
+      f1 = m1;
+      f2 = m2;
+      return;
+    }
     if (wm >= T(0)) { m0 = fl0; m1 = fl1; m2 = fl2; }
     if (wp <= T(0)) { m0 = fr0; m1 = fr1; m2 = fr2; }
     const T rstar = rl + am;
@@ -363,5 +393,38 @@ __host__ __device__ inline double value_of_key(unsigned long long k) {
   c.u = k - 1ull;
   return c.d;
 }
+
+// Per-lane CFL limit (hydro.py:66-68): min over the lane's zones of dx/(|vx| + cs) and
+// dy/(|vy| + cs), handed on as a key. FP64 keys every zone's length as the reference
+// divides. FP32 keeps the largest signal rate (|v| + cs) / d with a NaN-propagating
+// maximum (one v_maximum3 per zone, no divisions) and keys its reciprocal, in float64,
+// once per lane; a NaN state still yields key 0. Every FP32 kernel that forms the CFL
+// limit uses this accumulator, so all of them agree bit for bit.
+template <typename T> struct CflMin {
+  T dx, dy, cs;
+  unsigned long long k = ~0ull;
+  __device__ CflMin(T dx_, T dy_, T cs_) : dx(dx_), dy(dy_), cs(cs_) {}
+  __device__ __forceinline__ void add(T vx, T vy) {
+    const unsigned long long kx = key_of((double)dvd(dx, fabs(vx) + cs));
+    const unsigned long long ky = key_of((double)dvd(dy, fabs(vy) + cs));
+    k = kx < k ? kx : k;
+    k = ky < k ? ky : k;
+  }
+  __device__ __forceinline__ unsigned long long key() const { return k; }
+};
+template <> struct CflMin<float> {
+  float inv_dx, inv_dy, cs;
+  float smax = 0.0f;
+  __device__ CflMin(float dx_, float dy_, float cs_)
+      : inv_dx((float)(1.0 / (double)dx_)), inv_dy((float)(1.0 / (double)dy_)), cs(cs_) {}
+  __device__ __forceinline__ void add(float vx, float vy) {
+    const float rx = (fabsf(vx) + cs) * inv_dx, ry = (fabsf(vy) + cs) * inv_dy;
+    smax = __builtin_elementwise_maximum(__builtin_elementwise_maximum(smax, rx), ry);
+  }
+  __device__ __forceinline__ unsigned long long key() const {
+    // Rates are positive, so 0 means the lane met no zone.
+    return smax == 0.0f ? ~0ull : key_of(1.0 / (double)smax);
+  }
+};
 
 }  // namespace jx

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     for (int k = 0; k < R; ++k) w[k][v] = u[k][v] = T(0);
   }
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-  unsigned long long kmin = ~0ull;
+  CflMin<T> cfl(a.dx, a.dy, a.cs);
   T pf[NV];     // prefetched conserved values of the next row
   // Addresses are a wave-uniform 64-bit row base plus a 32-bit per-lane column offset.
   const size_t row_len = (size_t)nx * NV;
@@ -301,20 +301,20 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           const T* row = sW + v * BX;
-          xl[v] = ppm4_interface(row[tm2], row[tm1], row[t], row[tp1]);
+          xl[v] = ppm4_interface(row[tm2], row[tm1], w[S3][v], row[tp1]);
           sI[v * BX + t] = xl[v];
         }
         __syncthreads();   // B1b
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           xh[v] = sI[v * BX + tp1];
-          ppm4_limit(sW[v * BX + t], xl[v], xh[v]);
+          ppm4_limit(w[S3][v], xl[v], xh[v]);
         }
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           const T* row = sW + v * BX;
-          cell_edges<RECON>(row[tm2], row[tm1], row[t], row[tp1], row[tp2], a.eps, xl[v],
+          cell_edges<RECON>(row[tm2], row[tm1], w[S3][v], row[tp1], row[tp2], a.eps, xl[v],
                             xh[v]);
         }
       }
@@ -408,11 +408,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           un[0] = (un[0] < dfloor) ? dfloor : un[0];
           if (a.dtkey) {
             const T inv = dvd(T(1), un[0]);
-            const T lx = dvd<T>(a.dx, T(fabs(inv * un[1])) + a.cs);
-            const T ly = dvd<T>(a.dy, T(fabs(inv * un[2])) + a.cs);
-            const unsigned long long kx = key_of((double)lx), ky = key_of((double)ly);
-            kmin = kx < kmin ? kx : kmin;
-            kmin = ky < kmin ? ky : kmin;
+            cfl.add(inv * un[1], inv * un[2]);
           }
         }
 #pragma unroll
@@ -439,7 +435,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     block_reduce_store<T>(acc0, acc1, acc2, sred,
                           a.partial + 3 * (size_t)(blockIdx.y * gridDim.x + blockIdx.x));
   } else if (a.dtkey) {
-    block_min_key(kmin, reinterpret_cast<unsigned long long*>(sred), a.dtkey);
+    block_min_key(cfl.key(), reinterpret_cast<unsigned long long*>(sred), a.dtkey);
   }
 }
 
