@@ -26,7 +26,7 @@ __device__ __forceinline__ void load_prim(Ptr q, int nv, bool primitive, T dfloo
     vx = q[1];
     vy = q[2];
   } else {
-    r = (q[0] < dfloor) ? dfloor : q[0];
+    r = floor_to(q[0], dfloor);
     const T inv = dvd(T(1), r);
     vx = inv * q[1];
     vy = inv * q[2];
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(kAuxBlock) kick_apply_kernel(KickArgs<T> a) {
     const T rho = q[0];
     const T m0 = q[1] + rho * (acc0 * dt);
     const T m1 = q[2] + rho * (acc1 * dt);
-    const T rf = (rho < a.dfloor) ? a.dfloor : rho;
+    const T rf = floor_to(rho, a.dfloor);
     q[0] = rf;
     q[1] = m0;
     q[2] = m1;
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(kAuxBlock) recover_kernel(const T* u, T* uf, T
   for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (size_t)gridDim.x * blockDim.x) {
     const T* q = u + c * nv;
-    const T rho = (q[0] < dfloor) ? dfloor : q[0];
+    const T rho = floor_to(q[0], dfloor);
     const T inv = dvd(T(1), rho);
     if (uf) {
       uf[c * nv] = rho;

@@ -309,7 +309,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
         svr[v] = sv[v];
     }
     {
-      const T rho = (u[S0][0] < dfloor) ? dfloor : u[S0][0];
+      const T rho = floor_to(u[S0][0], dfloor);
       u[S0][0] = rho;
       const T inv = dvd(T(1), rho);
       w[S0][0] = rho;
@@ -457,7 +457,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
         T un[NV], us[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) us[v] = (k == 0) ? u[S3][v] : svr[v];
-        if (k > 0 && floor_saved) us[0] = (us[0] < dfloor) ? dfloor : us[0];
+        if (k > 0 && floor_saved) us[0] = floor_to(us[0], dfloor);
         if (has_delta) {
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
@@ -485,7 +485,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
           acc1 += (double)(un[0] * rf[0]);
           acc2 += (double)(un[0] * rf[1]);
         } else {
-          un[0] = (un[0] < dfloor) ? dfloor : un[0];
+          un[0] = floor_to(un[0], dfloor);
           if (want_key) {
             const T inv = dvd(T(1), un[0]);
             cfl.add(inv * un[1], inv * un[2]);
@@ -773,7 +773,7 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
       const T rho = q[0];
       const T m0 = q[1] + rho * (acc0 * dt);
       const T m1 = q[2] + rho * (acc1 * dt);
-      const T rf = (rho < I.dfloor) ? I.dfloor : rho;
+      const T rf = floor_to(rho, I.dfloor);
       q[0] = rf;
       q[1] = m0;
       q[2] = m1;

@@ -68,6 +68,13 @@ template <typename T> __device__ __forceinline__ T clamp_between(T e, T a, T b) 
   else return fmn(fmx(e, fmn(a, b)), fmx(a, b));
 }
 
+// Density floor (x < f) ? f : x (hydro.py:56). FP32 uses the NaN-propagating maximum, one
+// instruction with the same result for every x, NaN included (f > 0).
+template <typename T> __device__ __forceinline__ T floor_to(T x, T f) {
+  if constexpr (Fast<T>::value) return __builtin_elementwise_maximum(x, f);
+  else return (x < f) ? f : x;
+}
+
 // Division by a run constant d (dx, dy, cs, 4dx, 4dy).  EXACT: d is a power of two, so
 // x / d == x * (1/d) bit for bit and the multiply is used even in FP64.
 template <typename T, bool EXACT> struct Over {
@@ -89,16 +96,16 @@ __device__ __forceinline__ T ppm4_interface(T qm2, T qm1, T q, T qp1) {
 
 // PPM4 extremum and asymmetric-parabola limiter on clamped edges (fluxes.py:34-41).
 // FP32 uses the same limiter as two medians: with dl = lo - q and dh = hi - q,
-// dl' = med3(dl, -2 dh, 0). At an extremum (dl, dh of one sign, or one zero) dl and -2 dh
-// straddle zero and the median is 0 (lo = q). Otherwise both have the sign of dl and the
-// median is the smaller in magnitude: -2 dh exactly when |dl| >= 2 |dh|. Likewise for dh'.
-// Values agree with the select form up to one rounding of q + (lo - q).
+// lo' = med3(lo, q - 2 dh, q). At an extremum (dl, dh of one sign, or one zero) lo and
+// q - 2 dh lie on either side of q and the median is q. Otherwise both lie on the side
+// of lo and the median is the nearer to q: q - 2 dh exactly when |dl| >= 2 |dh|.
+// Likewise for hi'. Each result is one of the select form's operands (lo, q - 2 dh or q).
 template <typename T>
 __device__ __forceinline__ void ppm4_limit(T q, T& lo, T& hi) {
   const T dh = hi - q, dl = lo - q;
   if constexpr (Fast<T>::value) {
-    lo = q + __builtin_amdgcn_fmed3f(dl, T(-2) * dh, T(0));
-    hi = q + __builtin_amdgcn_fmed3f(dh, T(-2) * dl, T(0));
+    lo = __builtin_amdgcn_fmed3f(lo, q - T(2) * dh, q);
+    hi = __builtin_amdgcn_fmed3f(hi, q - T(2) * dl, q);
     return;
   }
   const bool ext = dh * dl >= T(0);
@@ -243,6 +250,49 @@ __device__ __forceinline__ T roe_average(T rl, T rr, T ul, T ur) {
   }
 }
 
+// FP32 Roe flux (fluxes.py:134-172) in fewer operations; equal to the reference form in
+// exact arithmetic:
+//  * it forms twice the flux and halves it at the end, grouping the left and right
+//    physical fluxes into one sum per component;
+//  * no supersonic selects: when every wave speed has one sign, |lambda| = +-lambda and
+//    the Roe property A(Ur - Ul) = Fr - Fl make the upwind sum equal fl (or fr) up to
+//    rounding;
+//  * the LLF fallback (negative intermediate or right density) is rare, so one wave-wide
+//    vote skips it.
+template <typename T>
+__device__ __forceinline__ void roe_fast(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
+                                         T& f0, T& f1, T& f2) {
+  const T cs = g.cs;
+  const T mnl = rl * ul, mnr = rr * ur, mtl = rl * tl;
+  T vn, vt;
+  roe_averages(rl, rr, ul, ur, tl, tr, vn, vt);
+  const T wm = vn - cs, wp = vn + cs;
+  const T drho = rr - rl, dmn = mnr - mnl, dmt = rr * tr - mtl;
+  const T h = T(0.5) * g.inv_cs;
+  const T am = drho * (T(0.5) + vn * h) - dmn * h;   // wave strengths, am + ap = drho
+  const T ap = drho - am;
+  const T ash = dmt - vt * drho;
+  const T s0 = mnl + mnr;
+  const T s1 = (mnl * ul + mnr * ur) + g.cs2 * (rl + rr);
+  const T s2 = mnl * tl + mnr * tr;
+  const T A = fabs(wm) * am, B = fabs(wp) * ap, AB = A + B;
+  T m0 = s0 - AB;
+  T m1 = (s1 - A * wm) - B * wp;
+  T m2 = (s2 - AB * vt) - fabs(vn) * ash;
+  const bool bad = (rl + am < T(0)) || (rr < T(0));
+  if (__builtin_amdgcn_ballot_w64(bad) != 0) {
+    if (bad) {
+      const T sp = fmx(fabs(ul) + cs, fabs(ur) + cs);
+      m0 = s0 - sp * drho;
+      m1 = s1 - sp * dmn;
+      m2 = s2 - sp * dmt;
+    }
+  }
+  f0 = T(0.5) * m0;
+  f1 = T(0.5) * m1;
+  f2 = T(0.5) * m2;
+}
+
 template <int RS, bool EXACT, typename T>
 __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
                                         T& f0, T& f1, T& f2) {
@@ -261,6 +311,25 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
   } else if constexpr (RS == RS_HLLE) {
     // fluxes.py:186-201
     const T vn = roe_average(rl, rr, ul, ur);
+    if constexpr (Fast<T>::value) {
+      // FP32: min(a - c, b - c) = min(a, b) - c exactly (rounding is monotone), and
+      // 0.5 (fl + fr) + wt (fl - fr) = (bp fl - bm fr) / (bp - bm) with the reference's
+      // shifted fluxes fl = F(Ul) - bm Ul, fr = F(Ur) - bp Ur, whose momentum and
+      // tangential parts are fl0 ul + cs^2 rl and fl0 tl.
+      const T al = fmn(vn, ul) - cs;
+      const T ar = fmx(vn, ur) + cs;
+      const T bm = (al < T(0)) ? al : T(-1e-20);
+      const T bp = (ar > T(0)) ? ar : T(1e-20);
+      const T fl0 = rl * (ul - bm), fr0 = rr * (ur - bp);
+      const T fl1 = fl0 * ul + cs2 * rl, fr1 = fr0 * ur + cs2 * rr;
+      const T fl2 = fl0 * tl, fr2 = fr0 * tr;
+      const T inv = dvd(T(1), bp - bm);
+      const T cl = bp * inv, cr = -(bm * inv);
+      f0 = cl * fl0 + cr * fr0;
+      f1 = cl * fl1 + cr * fr1;
+      f2 = cl * fl2 + cr * fr2;
+      return;
+    }
     const T al = fmn(vn - cs, ul - cs);
     const T ar = fmx(vn + cs, ur + cs);
     const T bm = (al < T(0)) ? al : T(-1e-20);
@@ -274,6 +343,10 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     f2 = T(0.5) * (fl2 + fr2) + wt * (fl2 - fr2);
   } else {
     // Roe with supersonic upwinding and the LLF fallback, fluxes.py:134-172
+    if constexpr (Fast<T>::value) {
+      roe_fast(rl, rr, ul, ur, tl, tr, g, f0, f1, f2);
+      return;
+    }
     const T mnl = rl * ul, mnr = rr * ur;
     const T mtl = rl * tl, mtr = rr * tr;
     const T fl0 = mnl, fl1 = mnl * ul + cs2 * rl, fl2 = mnl * tl;
@@ -303,26 +376,6 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     T m0 = (fa0 + cm) + cp;
     T m1 = (fa1 + cm * wm) + cp * wp;
     T m2 = ((fa2 + cm * vt) + csh) + cp * vt;
-    if constexpr (Fast<T>::value) {
-      // Supersonic faces: with every wave speed of one sign, |lambda| = +-lambda and the
-      // Roe property A(Ur - Ul) = Fr - Fl makes the sum above equal fl (or fr) up to
-      // rounding, so FP32 keeps the sum. The LLF fallback is rare: one wave-wide vote
-      // skips it.
-      const T rstar = rl + am;
-      const bool bad = (rstar < T(0)) || (rr < T(0));
-      if (__builtin_amdgcn_ballot_w64(bad) != 0) {
-        if (bad) {
-          const T sp = T(0.5) * fmx(fabs(ul) + cs, fabs(ur) + cs);
-          m0 = fa0 - sp * drho;
-          m1 = fa1 - sp * dmn;
-          m2 = fa2 - sp * dmt;
-        }
-      }
-      f0 = m0;
-      f1 = m1;
-      f2 = m2;
-      return;
-    }
     if (wm >= T(0)) { m0 = fl0; m1 = fl1; m2 = fl2; }
     if (wp <= T(0)) { m0 = fr0; m1 = fr1; m2 = fr2; }
     const T rstar = rl + am;

@@ -7,8 +7,9 @@
 //     and builds the y-face states, the y Riemann flux and (with viscosity) the y-face
 //     viscous flux without touching LDS for the stencil;
 //   * x direction: the row being updated is staged in LDS; the thread reconstructs its
-//     own cell, exchanges high edges and face fluxes through LDS (three barriers per row)
-//     and forms the x flux difference.  The 3+3 halo columns are computed, not stored.
+//     own cell, exchanges high edges (PPM4: interfaces) and face fluxes through LDS
+//     (three barriers per row) and forms the x flux difference.  The 3+3 halo columns
+//     are computed, not stored.
 // Face states and fluxes never leave the CU.  The update applies the RK row
 // (hydro.py:39-51), drag with the incoming primitives (hydro.py:122-125), the density
 // floor (hydro.py:54-57) and, in the epilogue, either the CFL min for the next timestep
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     }
     {
       // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
-      const T rho = (u[S0][0] < dfloor) ? dfloor : u[S0][0];
+      const T rho = floor_to(u[S0][0], dfloor);
       u[S0][0] = rho;
       const T inv = dvd(T(1), rho);
       w[S0][0] = rho;
@@ -216,6 +217,21 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       for (int v = 0; v < NV; ++v) sW[v * BX + t] = w[S3][v];
     }
     if (VISC ? (i >= 2) : (i >= 6)) __syncthreads();   // B1
+    // The x stencil of row r-3 is read now, so that the LDS latency overlaps the
+    // y-direction arithmetic below.
+    T xm2[NV], xm1[NV], xp1[NV], xp2[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) xm2[v] = xm1[v] = xp1[v] = xp2[v] = T(0);
+    if (i >= 6) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const T* row = sW + v * BX;
+        xm2[v] = row[tm2];
+        xm1[v] = row[tm1];
+        xp1[v] = row[tp1];
+        if constexpr (!SHARED) xp2[v] = row[tp2];
+      }
+    }
 
     // ---- y direction: edges of row r-2, flux at its low face -------------------------
     auto y_face = [&](T (&lo)[NV], T (&hi)[NV]) {
@@ -294,42 +310,43 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     // ---- x direction for row j = r-3, then the update --------------------------------
     if (i >= 6) {
       T xl[NV] = {}, xh[NV] = {};
+      T l[NV] = {};   // left state at the cell's low x face: the left neighbour's high edge
       if constexpr (SHARED) {
         // Each lane evaluates the clamped interface at its cell's low face once; the
         // high edge is the next lane's interface (same value the reference computes
-        // twice, see ppm4_interface).
+        // twice, see ppm4_interface). The lane also limits its left neighbour's edges
+        // (the same operation on the same values as that lane), which gives the left
+        // state without another exchange.
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          const T* row = sW + v * BX;
-          xl[v] = ppm4_interface(row[tm2], row[tm1], w[S3][v], row[tp1]);
+          xl[v] = ppm4_interface(xm2[v], xm1[v], w[S3][v], xp1[v]);
           sI[v * BX + t] = xl[v];
         }
         __syncthreads();   // B1b
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
+          T nlo = sI[v * BX + tm1], nhi = xl[v];
+          ppm4_limit(xm1[v], nlo, nhi);
+          l[v] = nhi;
           xh[v] = sI[v * BX + tp1];
           ppm4_limit(w[S3][v], xl[v], xh[v]);
         }
       } else {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const T* row = sW + v * BX;
-          cell_edges<RECON>(row[tm2], row[tm1], w[S3][v], row[tp1], row[tp2], a.eps, xl[v],
-                            xh[v]);
+        for (int v = 0; v < NV; ++v)
+          cell_edges<RECON>(xm2[v], xm1[v], w[S3][v], xp1[v], xp2[v], a.eps, xl[v], xh[v]);
+        if constexpr (floors_faces<RECON>()) {
+          xl[0] = fmx(xl[0], dfloor);
+          xh[0] = fmx(xh[0], dfloor);
         }
-      }
-      if constexpr (floors_faces<RECON>()) {
-        xl[0] = fmx(xl[0], dfloor);
-        xh[0] = fmx(xh[0], dfloor);
-      }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sH[v * BX + t] = xh[v];
-      __syncthreads();   // B2
-      T fx[NV] = {};
-      {
-        T l[NV] = {};
+        for (int v = 0; v < NV; ++v) sH[v * BX + t] = xh[v];
+        __syncthreads();   // B2
 #pragma unroll
         for (int v = 0; v < NV; ++v) l[v] = sH[v * BX + tm1];
+      }
+      T fx[NV] = {};
+      {
         T f0, f1, f2;
         riemann<RS, EXACT>(l[0], xl[0], l[1], xl[1], l[2], xl[2], gas, f0, f1, f2);
         fx[0] = f0;
@@ -374,7 +391,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         } else {
 #pragma unroll
           for (int v = 0; v < NV; ++v) us[v] = svr[v];
-          if (a.floor_saved) us[0] = (us[0] < dfloor) ? dfloor : us[0];
+          if (a.floor_saved) us[0] = floor_to(us[0], dfloor);
         }
         if (a.has_delta) {
           // RK4 saved-register accumulation (hydro.py:42-43)
@@ -405,7 +422,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           acc1 += (double)(un[0] * rf[0]);
           acc2 += (double)(un[0] * rf[1]);
         } else {
-          un[0] = (un[0] < dfloor) ? dfloor : un[0];
+          un[0] = floor_to(un[0], dfloor);
           if (a.dtkey) {
             const T inv = dvd(T(1), un[0]);
             cfl.add(inv * un[1], inv * un[2]);
