@@ -179,17 +179,20 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) u[S0][v] = pf[v];
       if (i + 1 < nit) {
-        int yn = (r + 1) % ny;
-        if (yn < 0) yn += ny;
+        int yn = r + 1;   // periodic row; only the 3-row halos wrap
+        if (yn < 0 || yn >= ny) {
+          yn %= ny;
+          if (yn < 0) yn += ny;
+        }
         const T* src = a.u_in + (size_t)yn * row_len;
 #pragma unroll
         for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
       }
     }
-    // Saved register of the row updated this iteration (pointwise, stages >= 2).
+    // Saved register of the row updated this iteration (pointwise, stages >= 2). Like
+    // the x stencil below it is only read under the condition that loads it, so it is
+    // left unset otherwise (no register clears per row).
     T svr[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) svr[v] = T(0);
     if (i >= 6 && !a.stage1) {
       const T* sv = a.u_sv + (size_t)(r - 3) * row_len;
 #pragma unroll
@@ -220,8 +223,6 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     // The x stencil of row r-3 is read now, so that the LDS latency overlaps the
     // y-direction arithmetic below.
     T xm2[NV], xm1[NV], xp1[NV], xp2[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) xm2[v] = xm1[v] = xp1[v] = xp2[v] = T(0);
     if (i >= 6) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
