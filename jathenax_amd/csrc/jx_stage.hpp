@@ -173,30 +173,31 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
     constexpr int S4 = (P + R - 4) % R;   // row r-4 (5-row ring only)
     const int r = y0 - 3 + i;
+    // Both global loads of the row are unconditional, so the memory-wait pass sees
+    // straight-line code and waits for each load alone. First the saved register of the
+    // row updated this iteration (pointwise; stage 1 reads its own input, during the
+    // warm-up any valid row), then the prefetch of row r+1 (past the end, the wrapped
+    // row). Loads return in order, so the update waits for the first while the prefetch
+    // stays in flight.
+    T svr[NV];
+    {
+      const T* sv = (a.stage1 ? a.u_in : a.u_sv) + (size_t)(i >= 6 ? r - 3 : y0) * row_len;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) svr[v] = sv[lane_in + v];
+    }
     {
       // Row r was fetched one iteration ago; issue row r+1 now so its latency overlaps
       // this row's arithmetic.
 #pragma unroll
       for (int v = 0; v < NV; ++v) u[S0][v] = pf[v];
-      if (i + 1 < nit) {
-        int yn = r + 1;   // periodic row; only the 3-row halos wrap
-        if (yn < 0 || yn >= ny) {
-          yn %= ny;
-          if (yn < 0) yn += ny;
-        }
-        const T* src = a.u_in + (size_t)yn * row_len;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
+      int yn = r + 1;   // periodic row; only the 3-row halos wrap
+      if (yn < 0 || yn >= ny) {
+        yn %= ny;
+        if (yn < 0) yn += ny;
       }
-    }
-    // Saved register of the row updated this iteration (pointwise, stages >= 2). Like
-    // the x stencil below it is only read under the condition that loads it, so it is
-    // left unset otherwise (no register clears per row).
-    T svr[NV];
-    if (i >= 6 && !a.stage1) {
-      const T* sv = a.u_sv + (size_t)(r - 3) * row_len;
+      const T* src = a.u_in + (size_t)yn * row_len;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) svr[v] = sv[lane_in + v];
+      for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
     }
     {
       // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     // The x stencil of row r-3 is read now, so that the LDS latency overlaps the
     // y-direction arithmetic below.
     T xm2[NV], xm1[NV], xp1[NV], xp2[NV];
-    if (i >= 6) {
+    {   // unconditional: before the first barrier these are in-bounds, unused LDS words
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const T* row = sW + v * BX;
