@@ -289,24 +289,28 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
     constexpr int S4 = (P + R - 4) % R;
     const int r = y0 - 3 + i;
+    // Unconditional loads, saved register first, then the prefetch of row r+1 (see the
+    // single-grid stage kernel): the update waits for the first only.
+    T svr[NV];
+    {
+      const gmem_t<T>* sv =
+          (has_sv ? g_sv : g_in) + ((size_t)(i >= 6 ? r - 3 : y0) * nx + xc) * nv;
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        svr[v] = sv[v];
+    }
 #pragma unroll
     for (int v = 0; v < NV; ++v) u[S0][v] = pf[v];
-    if (i + 1 < nit) {
-      int yn = (r + 1) % ny;
-      if (yn < 0) yn += ny;
+    {
+      int yn = r + 1;   // periodic row; only the 3-row halos wrap
+      if (yn < 0 || yn >= ny) {
+        yn %= ny;
+        if (yn < 0) yn += ny;
+      }
       const gmem_t<T>* src = g_in + ((size_t)yn * nx + xc) * nv;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         pf[v] = src[v];
-    }
-    T svr[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) svr[v] = T(0);
-    if (i >= 6 && has_sv) {
-      const gmem_t<T>* sv = g_sv + ((size_t)(r - 3) * nx + xc) * nv;
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-        svr[v] = sv[v];
     }
     {
       const T rho = floor_to(u[S0][0], dfloor);
@@ -327,6 +331,16 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
       for (int v = 0; v < NV; ++v) sW[v * 64 + t] = w[S3][v];
     }
     wave_sync();
+    // x stencil of row r-3, read before the y work (unused words during the warm-up).
+    T xm2[NV], xm1[NV], xp1[NV], xp2[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const T* row = sW + v * 64;
+      xm2[v] = row[tm2];
+      xm1[v] = row[tm1];
+      xp1[v] = row[tp1];
+      if constexpr (!SHARED) xp2[v] = row[tp2];
+    }
 
     // y direction: edges of row r-2, flux at its low face.
     auto y_face = [&](T (&lo)[NV], T (&hi)[NV]) {
@@ -391,40 +405,39 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     }
 
     if (i >= 6) {
-      T xl[NV], xh[NV];
+      T xl[NV], xh[NV], l[NV];
       if constexpr (SHARED) {
+        // Left states from the left neighbour's limiter, as in the single-grid kernel.
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          const T* row = sW + v * 64;
-          xl[v] = ppm4_interface(row[tm2], row[tm1], w[S3][v], row[tp1]);
+          xl[v] = ppm4_interface(xm2[v], xm1[v], w[S3][v], xp1[v]);
           sI[v * 64 + t] = xl[v];
         }
         wave_sync();
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
+          T nlo = sI[v * 64 + tm1], nhi = xl[v];
+          ppm4_limit(xm1[v], nlo, nhi);
+          l[v] = nhi;
           xh[v] = sI[v * 64 + tp1];
           ppm4_limit(w[S3][v], xl[v], xh[v]);
         }
       } else {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const T* row = sW + v * 64;
-          cell_edges<RECON>(row[tm2], row[tm1], w[S3][v], row[tp1], row[tp2], I.eps, xl[v],
-                            xh[v]);
+        for (int v = 0; v < NV; ++v)
+          cell_edges<RECON>(xm2[v], xm1[v], w[S3][v], xp1[v], xp2[v], I.eps, xl[v], xh[v]);
+        if (floors) {
+          xl[0] = fmx(xl[0], dfloor);
+          xh[0] = fmx(xh[0], dfloor);
         }
-      }
-      if (floors) {
-        xl[0] = fmx(xl[0], dfloor);
-        xh[0] = fmx(xh[0], dfloor);
-      }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sH[v * 64 + t] = xh[v];
-      wave_sync();
-      T fx[NV];
-      {
-        T l[NV];
+        for (int v = 0; v < NV; ++v) sH[v * 64 + t] = xh[v];
+        wave_sync();
 #pragma unroll
         for (int v = 0; v < NV; ++v) l[v] = sH[v * 64 + tm1];
+      }
+      T fx[NV];
+      {
         T f0, f1, f2;
         riemann_rt<T>(I.rs, l[0], xl[0], l[1], xl[1], l[2], xl[2], gas, f0, f1, f2);
         fx[0] = f0;
