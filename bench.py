@@ -177,8 +177,8 @@ def main():
         from jathenax_amd import tuning
         for dt_name, tdt in (("f32", torch.float32), ("f64", torch.float64)):
             U0 = torch.tensor(problems.smooth_wave(n, n), dtype=tdt, device="cuda")
-            choice, _ = tuning.tune_single(U0, 1.0 / n, 1.0 / n, 1.0, 1e-10, "rk3", steps=2,
-                                           repeats=2)
+            choice, _ = tuning.tune_single(U0, 1.0 / n, 1.0 / n, 1.0, 1e-10, "rk3", steps=4,
+                                           repeats=3)
             tiles[dt_name] = list(choice)
             del U0
         print(f"# bench: tuned tiles {tiles}", file=sys.stderr, flush=True)

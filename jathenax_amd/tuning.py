@@ -102,6 +102,8 @@ def tune_single(U, dx, dy, cs, dfloor, integrator, *, steps=3, repeats=2, candid
     was_pinned = ctx.pinned
     for cand in candidates or _single_candidates(p.nx, p.ny):
         ctx._apply_tiles(*cand)
+        # One untimed step first: the first launches after a tile change run cold.
+        _run._evolve_raw(U, dx, dy, cs, dfloor, integrator, 0.4, 1e9, 1, **kw)
         best = float("inf")
         for _ in range(repeats):
             torch.cuda.synchronize()
