@@ -72,7 +72,9 @@ def gpu_leg(jx, problems, n, dtype, steps, warmup, block_x, rows):
     assert cyc == steps, (cyc, steps)
     finite = bool(torch.isfinite(U).all())
     del U
-    return dict(wall=wall, launches=launches, kernel_ms=kernel_ms, finite=finite)
+    from jathenax_amd.hydro import run_params
+    used = ctx.tiles_for(run_params(U0, *args[:5]))
+    return dict(wall=wall, launches=launches, kernel_ms=kernel_ms, finite=finite, tiles=used)
 
 
 def ensemble_leg(n_inst):
@@ -139,14 +141,15 @@ def cpu_leg(problems, n):
     return n * n * 3 / sec, sec
 
 
-def read_traffic(n, dtype):
-    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary, if present."""
+def read_traffic(n, dtype, tiles):
+    """HBM bytes per stage launch for this grid, precision and tile shape, from the
+    committed rocprofv3 --pmc summary; None when that configuration was not measured."""
     f = ROOT / "profiles" / "stage_traffic.json"
     if not f.exists():
         return None
     try:
         data = json.loads(f.read_text())
-        return data.get(f"{n}x{n}_{dtype}")
+        return data.get(f"{n}x{n}_{dtype}_{tiles[0]}x{tiles[1]}")
     except (OSError, ValueError):
         return None
 
@@ -197,9 +200,10 @@ def main():
         "config": {"workload": f"{n}x{n} smooth wave, PPM4+Roe, RK3, nvar=3, CFL 0.4, "
                                f"device-resident time loop", "nx": n, "ny": n,
                    "reconstruct": "ppm4", "rsolver": "roe", "integrator": "rk3",
-                   "stages_per_step": stages, "tiles": tiles or "auto"},
+                   "stages_per_step": stages, "tiles": tiles or "auto",
+                   "tiles_used": list(main_leg["tiles"])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(n, a.dtype),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(n, a.dtype, main_leg["tiles"]),
                      "kernel": "jx::stage_kernel (fused RK stage)",
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3, "launches_timed": main_leg["launches"]},
