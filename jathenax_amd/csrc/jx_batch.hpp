@@ -293,8 +293,10 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     // single-grid stage kernel): the update waits for the first only.
     T svr[NV];
     {
-      const gmem_t<T>* sv =
-          (has_sv ? g_sv : g_in) + ((size_t)(i >= 6 ? r - 3 : y0) * nx + xc) * nv;
+      // Without a saved register (stage 1) the tile's first input row, which stays
+      // cached; during the warm-up likewise.
+      const gmem_t<T>* sv = has_sv ? g_sv + ((size_t)(i >= 6 ? r - 3 : y0) * nx + xc) * nv
+                                   : g_in + ((size_t)y0 * nx + xc) * nv;
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         svr[v] = sv[v];

@@ -175,13 +175,14 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     const int r = y0 - 3 + i;
     // Both global loads of the row are unconditional, so the memory-wait pass sees
     // straight-line code and waits for each load alone. First the saved register of the
-    // row updated this iteration (pointwise; stage 1 reads its own input, during the
-    // warm-up any valid row), then the prefetch of row r+1 (past the end, the wrapped
-    // row). Loads return in order, so the update waits for the first while the prefetch
-    // stays in flight.
+    // row updated this iteration (pointwise; during the warm-up, and in stage 1 where it
+    // is unused, the block's first input row, which stays cached), then the prefetch of
+    // row r+1 (past the end, the wrapped row). Loads return in order, so the update
+    // waits for the first while the prefetch stays in flight.
     T svr[NV];
     {
-      const T* sv = (a.stage1 ? a.u_in : a.u_sv) + (size_t)(i >= 6 ? r - 3 : y0) * row_len;
+      const T* sv = a.stage1 ? a.u_in + (size_t)y0 * row_len
+                             : a.u_sv + (size_t)(i >= 6 ? r - 3 : y0) * row_len;
 #pragma unroll
       for (int v = 0; v < NV; ++v) svr[v] = sv[lane_in + v];
     }
