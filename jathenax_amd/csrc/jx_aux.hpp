@@ -47,54 +47,128 @@ __device__ __forceinline__ unsigned long long max_key_of(double x) {
   return static_cast<unsigned long long>(__double_as_longlong(x));
 }
 
+// Work decomposition of timestep_kernel: each 64-lane wave owns a strip of kTsCols output
+// columns (lanes 1..62; lanes 0 and 63 carry the neighbouring columns) and a band of rows,
+// with about 8192 waves in all (32 per CU) so that other waves cover each row's load.
+constexpr int kTsCols = 62;
+struct TsTiling {
+  int nstrips, nbands, rows;
+};
+__host__ __device__ inline TsTiling ts_tiling(int nx, int ny) {
+  const int nstrips = (nx + kTsCols - 1) / kTsCols;
+  int nbands = (8192 + nstrips - 1) / nstrips;
+  nbands = nbands > ny ? ny : (nbands < 1 ? 1 : nbands);
+  const int rows = (ny + nbands - 1) / nbands;
+  return {nstrips, (ny + rows - 1) / rows, rows};
+}
+inline dim3 ts_grid(int nx, int ny) {
+  const TsTiling t = ts_tiling(nx, ny);
+  const int waves = t.nstrips * t.nbands;
+  return dim3((unsigned)((waves + kAuxBlock / 64 - 1) / (kAuxBlock / 64)));
+}
+
+// Each lane marches up its column. A cell's primitives are recovered once, when its row
+// enters; the row above and below stay in registers and the neighbouring columns come from
+// the adjacent lanes (one shuffle per value and row), so the Smagorinsky face stencils cost
+// no further loads. The face expressions are those of fluxes.py:227-245 in the reference
+// order, so the limits match the cell-by-cell form bit for bit.
 template <typename T>
 __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) {
   if (a.ctrl && a.ctrl->done) return;
-  __shared__ unsigned long long sred[2 * (kAuxBlock / 64)];
   const int nx = a.nx, ny = a.ny, nv = a.nv;
-  const size_t n = (size_t)nx * ny;
+  const TsTiling tl = ts_tiling(nx, ny);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * (kAuxBlock / 64) + (threadIdx.x >> 6)));
   CflMin<T> cfl(a.dx, a.dy, a.cs);
   unsigned long long kmax = 0ull;
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int j = (int)(c / nx), i = (int)(c % nx);
-    T r, vx, vy;
-    load_prim(a.q + c * nv, nv, a.primitive != 0, a.dfloor, r, vx, vy);
-    cfl.add(vx, vy);
-    if (a.smag_on) {
-      const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
-      const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
-      auto at = [&](int jj, int ii, T& u_, T& v_) {
-        T rr;
-        load_prim(a.q + ((size_t)jj * nx + ii) * nv, nv, a.primitive != 0, a.dfloor, rr, u_, v_);
-      };
-      T u_w, v_w, u_n, v_n, u_nw, v_nw, u_s, v_s, u_sw, v_sw, u_e, v_e, u_se, v_se;
-      at(j, im, u_w, v_w);
-      at(jp, i, u_n, v_n);
-      at(jp, im, u_nw, v_nw);
-      at(jm, i, u_s, v_s);
-      at(jm, im, u_sw, v_sw);
-      at(j, ip, u_e, v_e);
-      at(jm, ip, u_se, v_se);
-      // low x face (fluxes.py:227-238, axis=1)
-      {
-        const T ux = dvd(vx - u_w, a.dx);
-        const T vxn = dvd(vy - v_w, a.dx);
-        const T uy = dvd((u_n + u_nw) - (u_s + u_sw), a.four_dy);
-        const T vyt = dvd((v_n + v_nw) - (v_s + v_sw), a.four_dy);
-        const T nu = smag_nu(ux, vyt, vxn + uy, a.smag_c);
-        const unsigned long long k = max_key_of((double)nu);
-        kmax = k > kmax ? k : kmax;
+  if (wave < tl.nstrips * tl.nbands) {
+    const int strip = wave % tl.nstrips, band = wave / tl.nstrips;
+    const int x0 = strip * kTsCols;   // first output column
+    int xc = (x0 - 1 + lane) % nx;
+    if (xc < 0) xc += nx;
+    const bool out = lane >= 1 && lane <= kTsCols && x0 + lane - 1 < nx;
+    const int j0 = band * tl.rows, j1 = min(j0 + tl.rows, ny);
+    // Raw values of a row (periodic), and the primitives recovered from them.
+    auto fetch = [&](int j, T (&q)[3]) {
+      if (j < 0 || j >= ny) {
+        j %= ny;
+        if (j < 0) j += ny;
       }
-      // low y face (axis=0)
+      const T* src = a.q + ((size_t)j * nx + xc) * nv;
+      q[0] = src[0];
+      q[1] = src[1];
+      q[2] = src[2];
+    };
+    auto prim = [&](const T (&q)[3], T& u, T& v) {
+      T r;
+      load_prim(q, nv, a.primitive != 0, a.dfloor, r, u, v);
+    };
+    // Rows are fetched two ahead of their use, so their latency overlaps the work of the
+    // rows in between.
+    T q1[3], q2[3];
+    fetch(j0, q1);
+    fetch(j0 + 1, q2);
+    if (!a.smag_on) {
+      for (int j = j0; j < j1; ++j) {
+        T u, v;
+        prim(q1, u, v);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+        fetch(j + 2, q2);
+        if (out) cfl.add(u, v);
+      }
+    } else {
+      // Rows s = j-1, c = j of this column (o), the one to the left (l) and to the right
+      // (r): velocities vx (u) and vy (v).
+      T ou_s, ov_s, ou_c, ov_c;
       {
-        const T uy = dvd(vx - u_s, a.dy);
-        const T vyn = dvd(vy - v_s, a.dy);
-        const T uxt = dvd((u_e + u_se) - (u_w + u_sw), a.four_dx);
-        const T vxt = dvd((v_e + v_se) - (v_w + v_sw), a.four_dx);
-        const T nu = smag_nu(uxt, vyn, vxt + uy, a.smag_c);
-        const unsigned long long k = max_key_of((double)nu);
-        kmax = k > kmax ? k : kmax;
+        T q0[3];
+        fetch(j0 - 1, q0);
+        prim(q0, ou_s, ov_s);
+      }
+      prim(q1, ou_c, ov_c);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+      fetch(j0 + 2, q2);
+      T lu_s = __shfl_up(ou_s, 1), lv_s = __shfl_up(ov_s, 1);
+      T lu_c = __shfl_up(ou_c, 1), lv_c = __shfl_up(ov_c, 1);
+      T ru_s = __shfl_down(ou_s, 1), rv_s = __shfl_down(ov_s, 1);
+      T ru_c = __shfl_down(ou_c, 1), rv_c = __shfl_down(ov_c, 1);
+      for (int j = j0; j < j1; ++j) {
+        T ou_n, ov_n;   // row j+1
+        prim(q1, ou_n, ov_n);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+        fetch(j + 3, q2);
+        const T lu_n = __shfl_up(ou_n, 1), lv_n = __shfl_up(ov_n, 1);
+        const T ru_n = __shfl_down(ou_n, 1), rv_n = __shfl_down(ov_n, 1);
+        if (out) {
+          const T vx = ou_c, vy = ov_c;
+          cfl.add(vx, vy);
+          const T u_w = lu_c, v_w = lv_c, u_n = ou_n, v_n = ov_n, u_nw = lu_n, v_nw = lv_n;
+          const T u_s = ou_s, v_s = ov_s, u_sw = lu_s, v_sw = lv_s;
+          const T u_e = ru_c, v_e = rv_c, u_se = ru_s, v_se = rv_s;
+          {   // low x face (fluxes.py:227-238, axis=1)
+            const T ux = dvd(vx - u_w, a.dx);
+            const T vxn = dvd(vy - v_w, a.dx);
+            const T uy = dvd((u_n + u_nw) - (u_s + u_sw), a.four_dy);
+            const T vyt = dvd((v_n + v_nw) - (v_s + v_sw), a.four_dy);
+            const unsigned long long k = max_key_of((double)smag_nu(ux, vyt, vxn + uy, a.smag_c));
+            kmax = k > kmax ? k : kmax;
+          }
+          {   // low y face (axis=0)
+            const T uy = dvd(vx - u_s, a.dy);
+            const T vyn = dvd(vy - v_s, a.dy);
+            const T uxt = dvd((u_e + u_se) - (u_w + u_sw), a.four_dx);
+            const T vxt = dvd((v_e + v_se) - (v_w + v_sw), a.four_dx);
+            const unsigned long long k = max_key_of((double)smag_nu(uxt, vyn, vxt + uy, a.smag_c));
+            kmax = k > kmax ? k : kmax;
+          }
+        }
+        ou_s = ou_c; ov_s = ov_c; ou_c = ou_n; ov_c = ov_n;
+        lu_s = lu_c; lv_s = lv_c; lu_c = lu_n; lv_c = lv_n;
+        ru_s = ru_c; rv_s = rv_c; ru_c = ru_n; rv_c = rv_n;
       }
     }
   }
@@ -102,23 +176,27 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long o = __shfl_xor(kmin, off);
     kmin = o < kmin ? o : kmin;
-    const unsigned long long p = __shfl_xor(kmax, off);
-    kmax = p > kmax ? p : kmax;
+    const unsigned long long q = __shfl_xor(kmax, off);
+    kmax = q > kmax ? q : kmax;
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  // Waves of a block combine in LDS; then one update per block, skipped when the slot
+  // already holds a better value (slots only move one way, so a stale read can only cause
+  // a redundant atomic), which keeps thousands of blocks from queueing on two addresses.
+  __shared__ unsigned long long sred[2 * (kAuxBlock / 64)];
+  const int w = threadIdx.x >> 6;
   if (lane == 0) {
-    sred[wave] = kmin;
-    sred[nw + wave] = kmax;
+    sred[w] = kmin;
+    sred[kAuxBlock / 64 + w] = kmax;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long m = sred[0], x = sred[nw];
-    for (int w = 1; w < nw; ++w) {
-      m = sred[w] < m ? sred[w] : m;
-      x = sred[nw + w] > x ? sred[nw + w] : x;
+    for (int k = 1; k < kAuxBlock / 64; ++k) {
+      kmin = sred[k] < kmin ? sred[k] : kmin;
+      kmax = sred[kAuxBlock / 64 + k] > kmax ? sred[kAuxBlock / 64 + k] : kmax;
     }
-    atomicMin(a.dtkey, m);
-    if (a.smag_on) atomicMax(a.nukey, x);
+    if (kmin < *reinterpret_cast<volatile unsigned long long*>(a.dtkey)) atomicMin(a.dtkey, kmin);
+    if (a.smag_on && kmax > *reinterpret_cast<volatile unsigned long long*>(a.nukey))
+      atomicMax(a.nukey, kmax);
   }
 }
 

@@ -822,52 +822,98 @@ __global__ void __launch_bounds__(kBatchBlock) batch_kick_tiles_kernel(BatchTabs
   }
 }
 
-// CFL and Smagorinsky limits over one tile of an instance's current state A.
+// CFL and Smagorinsky limits over one tile of an instance's current state A. Lane t follows
+// column x0 - 1 + t (periodic) and marches up the tile's rows; lanes 1..kTileCols are the
+// tile's columns, lanes 0 and kTileCols + 1 their neighbours. A cell's primitives are
+// recovered once, as in the single-grid timestep kernel, and the face expressions are the
+// same, so both give the same limits bit for bit.
 template <typename T>
 __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl) {
   const int t = threadIdx.x & 63;
-  const int i = tl.x0 + t;
-  const bool on = t < kTileCols && i < I.nx;
   const int nx = I.nx, ny = I.ny, nv = I.nv;
+  int xc = (tl.x0 - 1 + t) % nx;
+  if (xc < 0) xc += nx;
+  const bool on = t >= 1 && t <= kTileCols && tl.x0 + t - 1 < nx;
   CflMin<T> cfl(I.dx, I.dy, I.cs);
   unsigned long long kmax = 0ull;
   const gmem_t<T>* A = gmem(static_cast<const T*>(I.A));
-  const int yend = on ? min(tl.y0 + tl.rows, ny) : tl.y0;
-  for (int j = tl.y0; j < yend; ++j) {
-    T r, vx, vy;
-    load_prim(A + ((size_t)j * nx + i) * nv, nv, false, I.dfloor, r, vx, vy);
-    cfl.add(vx, vy);
-    if (I.smag_on) {
-      const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
-      const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
-      auto at = [&](int jj, int ii, T& u_, T& v_) {
-        T rr;
-        load_prim(A + ((size_t)jj * nx + ii) * nv, nv, false, I.dfloor, rr, u_, v_);
-      };
-      T u_w, v_w, u_n, v_n, u_nw, v_nw, u_s, v_s, u_sw, v_sw, u_e, v_e, u_se, v_se;
-      at(j, im, u_w, v_w);
-      at(jp, i, u_n, v_n);
-      at(jp, im, u_nw, v_nw);
-      at(jm, i, u_s, v_s);
-      at(jm, im, u_sw, v_sw);
-      at(j, ip, u_e, v_e);
-      at(jm, ip, u_se, v_se);
-      {
-        const T ux = dvd(vx - u_w, I.dx);
-        const T vxn = dvd(vy - v_w, I.dx);
-        const T uy = dvd((u_n + u_nw) - (u_s + u_sw), I.four_dy);
-        const T vyt = dvd((v_n + v_nw) - (v_s + v_sw), I.four_dy);
-        const unsigned long long k = max_key_of((double)smag_nu(ux, vyt, vxn + uy, I.smag_c));
-        kmax = k > kmax ? k : kmax;
+  const int j0 = tl.y0, j1 = min(tl.y0 + tl.rows, ny);
+  auto fetch = [&](int j, T (&q)[3]) {
+    if (j < 0 || j >= ny) {
+      j %= ny;
+      if (j < 0) j += ny;
+    }
+    const gmem_t<T>* src = A + ((size_t)j * nx + xc) * nv;
+    q[0] = src[0];
+    q[1] = src[1];
+    q[2] = src[2];
+  };
+  auto prim = [&](const T (&q)[3], T& u, T& v) {
+    T r;
+    load_prim(q, nv, false, I.dfloor, r, u, v);
+  };
+  // Rows are fetched two ahead of their use (see the single-grid kernel).
+  T q1[3], q2[3];
+  fetch(j0, q1);
+  fetch(j0 + 1, q2);
+  if (!I.smag_on) {
+    for (int j = j0; j < j1; ++j) {
+      T u, v;
+      prim(q1, u, v);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+      fetch(j + 2, q2);
+      if (on) cfl.add(u, v);
+    }
+  } else {
+    T ou_s, ov_s, ou_c, ov_c;
+    {
+      T q0[3];
+      fetch(j0 - 1, q0);
+      prim(q0, ou_s, ov_s);
+    }
+    prim(q1, ou_c, ov_c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+    fetch(j0 + 2, q2);
+    T lu_s = __shfl_up(ou_s, 1), lv_s = __shfl_up(ov_s, 1);
+    T lu_c = __shfl_up(ou_c, 1), lv_c = __shfl_up(ov_c, 1);
+    T ru_s = __shfl_down(ou_s, 1), rv_s = __shfl_down(ov_s, 1);
+    T ru_c = __shfl_down(ou_c, 1), rv_c = __shfl_down(ov_c, 1);
+    for (int j = j0; j < j1; ++j) {
+      T ou_n, ov_n;
+      prim(q1, ou_n, ov_n);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+      fetch(j + 3, q2);
+      const T lu_n = __shfl_up(ou_n, 1), lv_n = __shfl_up(ov_n, 1);
+      const T ru_n = __shfl_down(ou_n, 1), rv_n = __shfl_down(ov_n, 1);
+      if (on) {
+        const T vx = ou_c, vy = ov_c;
+        cfl.add(vx, vy);
+        const T u_w = lu_c, v_w = lv_c, u_n = ou_n, v_n = ov_n, u_nw = lu_n, v_nw = lv_n;
+        const T u_s = ou_s, v_s = ov_s, u_sw = lu_s, v_sw = lv_s;
+        const T u_e = ru_c, v_e = rv_c, u_se = ru_s, v_se = rv_s;
+        {
+          const T ux = dvd(vx - u_w, I.dx);
+          const T vxn = dvd(vy - v_w, I.dx);
+          const T uy = dvd((u_n + u_nw) - (u_s + u_sw), I.four_dy);
+          const T vyt = dvd((v_n + v_nw) - (v_s + v_sw), I.four_dy);
+          const unsigned long long k = max_key_of((double)smag_nu(ux, vyt, vxn + uy, I.smag_c));
+          kmax = k > kmax ? k : kmax;
+        }
+        {
+          const T uy = dvd(vx - u_s, I.dy);
+          const T vyn = dvd(vy - v_s, I.dy);
+          const T uxt = dvd((u_e + u_se) - (u_w + u_sw), I.four_dx);
+          const T vxt = dvd((v_e + v_se) - (v_w + v_sw), I.four_dx);
+          const unsigned long long k = max_key_of((double)smag_nu(uxt, vyn, vxt + uy, I.smag_c));
+          kmax = k > kmax ? k : kmax;
+        }
       }
-      {
-        const T uy = dvd(vx - u_s, I.dy);
-        const T vyn = dvd(vy - v_s, I.dy);
-        const T uxt = dvd((u_e + u_se) - (u_w + u_sw), I.four_dx);
-        const T vxt = dvd((v_e + v_se) - (v_w + v_sw), I.four_dx);
-        const unsigned long long k = max_key_of((double)smag_nu(uxt, vyn, vxt + uy, I.smag_c));
-        kmax = k > kmax ? k : kmax;
-      }
+      ou_s = ou_c; ov_s = ov_c; ou_c = ou_n; ov_c = ov_n;
+      lu_s = lu_c; lv_s = lv_c; lu_c = lu_n; lv_c = lv_n;
+      ru_s = ru_c; rv_s = rv_c; ru_c = ru_n; rv_c = rv_n;
     }
   }
   unsigned long long kmin = cfl.key();
@@ -877,9 +923,11 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
     const unsigned long long p = __shfl_xor(kmax, off);
     kmax = p > kmax ? p : kmax;
   }
-  if (t == 0) {
-    atomicMin(&C.c.dtkey, kmin);
-    if (I.smag_on) atomicMax(&C.c.nukey, kmax);
+  if (t == 0) {   // skip updates that cannot change the slot (see timestep_kernel)
+    if (kmin < *reinterpret_cast<volatile unsigned long long*>(&C.c.dtkey))
+      atomicMin(&C.c.dtkey, kmin);
+    if (I.smag_on && kmax > *reinterpret_cast<volatile unsigned long long*>(&C.c.nukey))
+      atomicMax(&C.c.nukey, kmax);
   }
 }
 

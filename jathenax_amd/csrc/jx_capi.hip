@@ -510,8 +510,8 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
   }
   const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
   // Initial timestep from the caller's state (run.py:42-46).
-  hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0, c->stream,
-                     timestep_args<T>(c, p, in, 0, c->ctrl));
+  hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
+                     c->stream, timestep_args<T>(c, p, in, 0, c->ctrl));
   auto one_step = [&](bool first) -> int {
     hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
     if (forced)
@@ -520,7 +520,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     int r = launch_step<T>(c, s, in, out, first, raw, !smag, hook);
     if (r) return r;
     if (smag)
-      hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
+      hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
                          c->stream, timestep_args<T>(c, p, out, 0, c->ctrl));
     return JX_OK;
   };
@@ -555,8 +555,9 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
       int r = launch_step<T>(c, s, in, out, false, raw, !smag, hook);
       if (r) return r;
       if (smag)
-        hipLaunchKernelGGL(jx::timestep_kernel<T>, aux_grid(n), dim3(jx::kAuxBlock), 0,
-                           c->stream, timestep_args<T>(c, p, out, 0, c->ctrl));
+        hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny),
+                           dim3(jx::kAuxBlock), 0, c->stream,
+                           timestep_args<T>(c, p, out, 0, c->ctrl));
     }
     for (long long k = 1; k < chunk; ++k) {
       rc = one_step(false);
@@ -803,15 +804,13 @@ int jx_isothermal_timestep(jx_ctx* c, const jx_run_params* p, const void* W, dou
   if (!c || !dt_out) return JX_EINVAL;
   if (!p || !W) return fail(c, JX_EINVAL, "params and W must not be NULL");
   if (p->nx < 1 || p->ny < 1 || p->nvar < 3) return fail(c, JX_EINVAL, "bad shape");
-  const size_t n = (size_t)p->nx * p->ny;
+  const dim3 grid = jx::ts_grid(p->nx, p->ny);
   hipLaunchKernelGGL(jx::init_ctrl_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl);
   if (p->dtype == 0)
-    hipLaunchKernelGGL(jx::timestep_kernel<double>, aux_grid(n), dim3(jx::kAuxBlock), 0,
-                       c->stream,
+    hipLaunchKernelGGL(jx::timestep_kernel<double>, grid, dim3(jx::kAuxBlock), 0, c->stream,
                        timestep_args<double>(c, p, static_cast<const double*>(W), 1, nullptr));
   else
-    hipLaunchKernelGGL(jx::timestep_kernel<float>, aux_grid(n), dim3(jx::kAuxBlock), 0,
-                       c->stream,
+    hipLaunchKernelGGL(jx::timestep_kernel<float>, grid, dim3(jx::kAuxBlock), 0, c->stream,
                        timestep_args<float>(c, p, static_cast<const float*>(W), 1, nullptr));
   JX_HIP(c, hipGetLastError());
   JX_HIP(c, hipMemcpyAsync(c->host_ctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost,

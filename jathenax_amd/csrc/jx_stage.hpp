@@ -90,7 +90,8 @@ __device__ __forceinline__ void block_min_key(unsigned long long k, unsigned lon
     const int nw = (blockDim.x + 63) >> 6;
     unsigned long long m = sred[0];
     for (int w = 1; w < nw; ++w) m = sred[w] < m ? sred[w] : m;
-    atomicMin(slot, m);
+    // The slot only decreases, so a stale read can only cause a redundant atomic.
+    if (m < *reinterpret_cast<volatile unsigned long long*>(slot)) atomicMin(slot, m);
   }
 }
 
