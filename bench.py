@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--serve", type=int, default=16,
                     help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, 1/s")
+    ap.add_argument("--configs", type=int, default=20,
+                    help="steps per workload in the KH 1024^2 / turbulence 4096^2 leg (0 = skip)")
     return ap.parse_args()
 
 
@@ -112,6 +114,42 @@ def ensemble_leg(n_inst):
             "workload": "ensemble_mix seed 20261203: sizes log-uniform 64..512 per axis, KH "
                         "(tlim 0.1) / driven turbulence (tlim 0.05), recon plm/ppm4/wenoz, "
                         "rsolver llf/hlle/roe, rk2/rk3, FP32, one batched engine"}
+
+
+def configs_leg(steps):
+    """Throughput of the other single-GPU workloads, device-resident loop, `steps` steps each
+    after one warm-up step: Kelvin-Helmholtz 1024^2 with tracer (PPM4+Roe, RK3) in FP32 and
+    FP64, and driven turbulence 4096^2 FP32 (WENO-Z+HLLE, RK2, CFL 0.3, forcing refreshed
+    every step, viscosity, Smagorinsky and drag)."""
+    import torch
+    from jathenax_amd import problems, run, workloads
+    out = {}
+    U = problems.kelvin_helmholtz_tanh(1024, 1024)
+    for name, tdt in (("kh_1024_f32", torch.float32), ("kh_1024_f64", torch.float64)):
+        U0 = torch.tensor(U, dtype=tdt, device="cuda")
+        args = (1 / 1024, 1 / 1024, 1.0, 1e-10, "rk3", 0.4, 1e9)
+        run.evolve_isothermal(U0, *args, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.evolve_isothermal(U0, *args, steps)
+        torch.cuda.synchronize()
+        w = time.perf_counter() - t0
+        out[name] = {"zone_updates_per_s": 1024 * 1024 * 3 * steps / w,
+                     "ms_per_step": w / steps * 1e3}
+    s = workloads.turbulence_spec(4096, 4096, 0.01, 1618, 5e-5, "rk2", "wenoz", "hlle", cfl=0.3,
+                                  dtype=np.float32)
+    U0 = torch.tensor(s["U"], dtype=torch.float32, device="cuda")
+    args = (s["dx"], s["dy"], s["cs"], s["dfloor"], s["integrator"], s["cfl"], s["tlim"])
+    run.evolve_isothermal(U0, *args, 1, **s["kw"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.evolve_isothermal(U0, *args, steps, **s["kw"])
+    torch.cuda.synchronize()
+    w = time.perf_counter() - t0
+    out["turbulence_4096_f32"] = {"zone_updates_per_s": 4096 * 4096 * 2 * steps / w,
+                                  "ms_per_step": w / steps * 1e3}
+    out["steps"] = steps
+    return out
 
 
 def serving_leg(n_req, rate):
@@ -216,6 +254,9 @@ def main():
         out["fp64"] = {"value": zu64, "ms_per_step": f64["wall"] / K * 1e3,
                        "avg_launch_ms": f64["kernel_ms"] / max(1, f64["launches"]),
                        "fp32_over_fp64": zu / zu64}
+    if a.configs > 0:
+        print("# bench: other workloads", file=sys.stderr, flush=True)
+        out["configs"] = configs_leg(a.configs)
     if a.ensemble > 0:
         print(f"# bench: ensemble leg ({a.ensemble} instances)", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_leg(a.ensemble)
