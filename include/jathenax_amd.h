@@ -237,6 +237,11 @@ int jx_batch_checkpoint(jx_batch* b, int id, void* U_out, jx_loop_state* state_o
 int jx_batch_remove(jx_batch* b, int id);
 /* Run at least `rounds` rounds; returns the number of still-active instances. */
 int jx_batch_run(jx_batch* b, int rounds, int* active_out);
+/* jx_batch_run in two halves: enqueue the rounds on the batch's stream and return at once,
+ * then wait for them.  Other streams (for example a single-grid run) can work on the GPU in
+ * between.  No counterpart in the reference (batching is new). */
+int jx_batch_launch(jx_batch* b, int rounds);
+int jx_batch_wait(jx_batch* b, int* active_out);
 /* Per slot: done (1), running (0) or empty (-1), time and cycle. */
 int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap);
 /* Copy an instance's state to U_out (may be NULL) and return (time, last dt, cycle). */

@@ -23,6 +23,8 @@ struct BatchBase {
   virtual int checkpoint(int id, void* U_out, jx_loop_state* state) = 0;
   virtual int remove(int id) = 0;
   virtual int run(int rounds, int* active) = 0;
+  virtual int launch(int rounds) = 0;
+  virtual int wait(int* active) = 0;
   virtual int poll(int* done, double* time, int* cycle, int cap) = 0;
   virtual int result(int id, void* U_out, double* time, double* dt, int* cycle) = 0;
   virtual int state_ptr(int id, void** p) = 0;
@@ -230,6 +232,11 @@ template <typename T> class Batch : public BatchBase {
   }
 
   int run(int rounds, int* active) override {
+    const int rc = launch(rounds);
+    return rc ? rc : wait(active);
+  }
+
+  int launch(int rounds) override {
     if (rounds < 0) return fail("run: rounds must be >= 0");
     if (use_graph_) {
       if (!graph_exec_) {
@@ -243,6 +250,10 @@ template <typename T> class Batch : public BatchBase {
       for (int k = 0; k < rounds; ++k) launch_round();
     }
     if (hipGetLastError() != hipSuccess) return fail("round launch failed");
+    return JX_OK;
+  }
+
+  int wait(int* active) override {
     if (hipMemcpyAsync(h_meta_, d_meta_, sizeof(int) * 8, hipMemcpyDeviceToHost, stream_) !=
             hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)

@@ -1142,6 +1142,14 @@ int jx_batch_run(jx_batch* b, int rounds, int* active_out) {
   return b ? b->impl->run(rounds, active_out) : JX_EINVAL;
 }
 
+int jx_batch_launch(jx_batch* b, int rounds) {
+  return b ? b->impl->launch(rounds) : JX_EINVAL;
+}
+
+int jx_batch_wait(jx_batch* b, int* active_out) {
+  return b ? b->impl->wait(active_out) : JX_EINVAL;
+}
+
 int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap) {
   return b ? b->impl->poll(done, time, cycle, cap) : JX_EINVAL;
 }

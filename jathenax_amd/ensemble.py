@@ -150,6 +150,17 @@ class Ensemble:
         self._check(_lib.load().jx_batch_run(self.handle, int(rounds), ctypes.byref(act)), "run")
         return act.value
 
+    def launch(self, rounds):
+        """Enqueue `rounds` rounds on the engine's stream and return at once; wait() ends
+        them.  Work on other streams overlaps them on the GPU."""
+        self._check(_lib.load().jx_batch_launch(self.handle, int(rounds)), "launch")
+
+    def wait(self):
+        """Wait for the launched rounds; returns the active count."""
+        act = ctypes.c_int()
+        self._check(_lib.load().jx_batch_wait(self.handle, ctypes.byref(act)), "wait")
+        return act.value
+
     def run_until_done(self, chunk=48, max_rounds=10 ** 9):
         done = 0
         while done < max_rounds:

@@ -14,6 +14,14 @@ running, the lowest-priority running request is checkpointed (state and loop sta
 removed and requeued; resumed later, it continues bit for bit, so pre-emption changes only
 latency.
 
+Large requests: a request of at least `big_zones` cells runs on the single-grid fused
+kernel (the device-resident loop, `big_chunk` steps per server iteration) on a stream of its
+own, while the batch's rounds run on the engine's stream: one grid of that size fills the
+GPU better alone than as tiles among others. At most `max_big` run at once; the rest wait
+without blocking smaller requests. A request's path depends only on its size, so its result
+is the same alone as in any mix. The two paths agree bit for bit on unforced runs and to
+round-off on forced ones (their kick sums are reduced in different orders).
+
 Failure handling: a request whose time or state is non-finite, or that exceeds its round
 budget, is evicted with an error status instead of blocking the batch.
 
@@ -49,6 +57,8 @@ class Request:
     checkpoint: object = None      # (U on device, run.LoopState) while pre-empted
     preemptions: int = 0
     info: dict = field(default_factory=dict)
+    path: str = "batch"            # "batch" or "grid" (single-grid kernel, large requests)
+    state: object = None           # grid path: (U on device, run.LoopState) between chunks
 
     @property
     def latency(self):
@@ -57,15 +67,24 @@ class Request:
 
 class Server:
     def __init__(self, pool_bytes=None, max_instances=256, dtype=torch.float32,
-                 rounds_per_poll=6, keep_results=True, graph=True, preempt=True):
+                 rounds_per_poll=6, keep_results=True, graph=True, preempt=True,
+                 big_zones=1 << 20, big_chunk=16, max_big=1, big_rounds=4):
         self.ens = Ensemble(dtype=dtype, max_instances=max_instances, pool_bytes=pool_bytes,
                             graph=graph, graph_rounds=rounds_per_poll)
         self.dtype = dtype
         self.rounds_per_poll = rounds_per_poll
         self.keep_results = keep_results
         self.preempt = preempt
+        self.big_zones = big_zones          # 0: every request takes the batch path
+        self.big_chunk = big_chunk
+        self.max_big = max_big
+        # While large requests run, each server iteration launches big_rounds times as many
+        # batch rounds, so the batch keeps the GPU busy during the large requests' chunks.
+        self.big_rounds = big_rounds
         self.queue = deque()
         self.running = {}
+        self.grid_running = {}
+        self.grid_stream = torch.cuda.Stream()
         self.finished = []
         self.clock0 = None
 
@@ -102,16 +121,39 @@ class Server:
         return self.ens.footprint(nx, ny, nv, forced="forcing" in spec["kw"]
                                   and spec["kw"]["forcing"] is not None)
 
+    def _is_big(self, req):
+        ny, nx = req.spec["U"].shape[:2]
+        return bool(self.big_zones) and nx * ny >= self.big_zones
+
     def _next(self):
-        """The highest-priority request that has arrived (the queue is sorted)."""
+        """The highest-priority arrived request (the queue is sorted) that has a path free:
+        large requests wait for a single-grid slot without blocking the others."""
         now = self.now()
-        return next((r for r in self.queue if r.arrival <= now), None)
+        grid_full = len(self.grid_running) >= self.max_big
+        return next((r for r in self.queue if r.arrival <= now
+                     and not (grid_full and self._is_big(r))), None)
+
+    def _admit_grid(self, req):
+        self.queue.remove(req)
+        s = req.spec
+        if req.checkpoint is not None:
+            req.state, req.checkpoint = req.checkpoint, None
+        else:
+            req.state = (torch.as_tensor(s["U"], dtype=self.dtype, device="cuda"), None)
+        req.path = "grid"
+        if math.isnan(req.admitted):
+            req.admitted = self.now()
+        req.status = "running"
+        self.grid_running[req.rid] = req
 
     def _admit(self):
         while True:
             req = self._next()
             if req is None:
                 return
+            if self._is_big(req):
+                self._admit_grid(req)
+                continue
             if not self._fits(req):
                 if self.preempt and self._preempt_for(req):
                     pass
@@ -139,11 +181,11 @@ class Server:
             req.status = "running"
             self.running[req.iid] = req
 
-    def _retire(self):
+    def _retire(self, rounds):
         done, times, cycles = self.ens.poll()
         for iid in list(self.running):
             req = self.running[iid]
-            req.rounds += self.rounds_per_poll
+            req.rounds += rounds
             bad = not math.isfinite(times[iid])
             if done[iid] == 1 or bad or req.rounds > req.max_rounds:
                 res = self.ens.result(iid)
@@ -160,17 +202,55 @@ class Server:
                 del self.running[iid]
                 self.finished.append(req)
 
+    def _step_grid(self):
+        """One chunk of steps for every large request, on the grid stream (the batch's
+        launched rounds run meanwhile on the engine's stream)."""
+        from .run import evolve_resumable
+        self.grid_stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self.grid_stream):
+            for rid in list(self.grid_running):
+                req = self.grid_running[rid]
+                s = req.spec
+                U, st = req.state
+                target = min(s["nlim"], (st.cycle if st else 0) + self.big_chunk)
+                U, st = evolve_resumable(U, s["dx"], s["dy"], s["cs"], s["dfloor"],
+                                         s["integrator"], s["cfl"], s["tlim"], target, state=st,
+                                         **s["kw"])
+                req.state = (U, st)
+                req.rounds += self.big_chunk
+                bad = not math.isfinite(st.time)
+                if bad or not (st.time < s["tlim"] and st.cycle < s["nlim"]) \
+                        or req.rounds > req.max_rounds:
+                    req.finished = self.now()
+                    req.info["cycles"] = int(st.cycle)
+                    if bad or not torch.isfinite(U).all():
+                        req.status, req.info["error"] = "error", "non-finite state"
+                    elif st.time < s["tlim"] and st.cycle < s["nlim"]:
+                        req.status, req.info["error"] = "error", "round budget exceeded"
+                    else:
+                        req.status = "done"
+                    req.result = (U, st.time, st.dt, st.cycle) if self.keep_results else None
+                    req.state = None
+                    del self.grid_running[rid]
+                    self.finished.append(req)
+        self.grid_stream.synchronize()
+
     def serve(self, requests, timeout=None):
         """Run until every request has finished (or `timeout` seconds elapsed)."""
         for r in requests:
             self.submit(r)
         self.clock0 = time.perf_counter()
-        while self.queue or self.running:
+        while self.queue or self.running or self.grid_running:
             self._admit()
+            rounds = self.rounds_per_poll * (self.big_rounds if self.grid_running else 1)
             if self.running:
-                self.ens.run(self.rounds_per_poll)
-                self._retire()
-            elif self.queue:
+                self.ens.launch(rounds)
+            if self.grid_running:
+                self._step_grid()
+            if self.running:
+                self.ens.wait()
+                self._retire(rounds)
+            elif not self.grid_running and self.queue:
                 wait = min(r.arrival for r in self.queue) - self.now()
                 if wait > 0:
                     time.sleep(min(wait, 0.01))

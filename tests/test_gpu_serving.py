@@ -57,3 +57,35 @@ def test_poisoned_request_is_evicted():
     out = {r.rid: r for r in Server(pool_bytes=1 << 28, max_instances=8).serve(reqs)}
     assert out[1].status == "error"
     assert all(out[k].status == "done" for k in (0, 2, 3))
+
+
+def test_large_requests_on_the_single_grid_path():
+    """Requests at or above big_zones cells run on the single-grid kernel beside the batch.
+    Served alone under the same threshold a request takes the same path and gives the same
+    result bit for bit; against the batched path the result agrees bit for bit when
+    unforced and to FP32 round-off when forced (kick sums reduced in another order)."""
+    from jathenax_amd.serving import Request, Server
+    reqs = _requests(n=6, seed=11)
+    sizes = sorted(r.spec["U"].shape[0] * r.spec["U"].shape[1] for r in reqs)
+    big = sizes[3]
+    out = Server(pool_bytes=1 << 28, max_instances=8, big_zones=big, max_big=1,
+                 big_chunk=5).serve(reqs)
+    assert all(r.status == "done" for r in out), [(r.rid, r.status, r.info) for r in out]
+    assert {r.path for r in out} == {"grid", "batch"}
+    for r in out:
+        alone = Server(pool_bytes=1 << 28, max_instances=2, big_zones=big, big_chunk=7).serve(
+            [Request(rid=r.rid, spec=r.spec, arrival=0.0)])[0]
+        assert alone.status == "done" and alone.path == r.path
+        np.testing.assert_array_equal(alone.result[0].cpu().numpy(), r.result[0].cpu().numpy())
+        assert alone.result[1:] == r.result[1:]
+        if r.path != "grid":
+            continue
+        batch = Server(pool_bytes=1 << 28, max_instances=2, big_zones=0).serve(
+            [Request(rid=r.rid, spec=r.spec, arrival=0.0)])[0]
+        a, b = batch.result[0].cpu().numpy(), r.result[0].cpu().numpy()
+        if r.spec["kw"].get("forcing") is None:
+            np.testing.assert_array_equal(a, b)
+            assert batch.result[1:] == r.result[1:]
+        else:
+            assert batch.result[3] == r.result[3]
+            assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b))
