@@ -211,14 +211,16 @@ bool is_pow2(double x) {
   return std::frexp(x, &e) == 0.5 && x >= 1e-300 && x <= 1e300;
 }
 
-// Strip width: the whole row when it fits one block, else 256 threads.  Rows per block:
-// given, or sized so that the grid is one full wave of resident blocks (no tail wave);
-// at least 32 rows so the 6-row warm-up of each block stays small.
+// Strip width: the whole row when it fits one block, else 128 threads (two waves per
+// barrier).  Rows per block: given, or sized so that the grid is one full wave of resident
+// blocks (no tail wave), with at least 12 rows: on grids too small to fill the GPU, more
+// blocks beat the cost of each block's 6-row warm-up (1024^2: 12 rows run twice as fast
+// as 32).
 template <typename T>
 void pick_tiles(const jx_ctx* c, const jx_run_params* p, bool visc, bool exact, int* bx,
                 int* rows) {
   int b = c->block_x;
-  if (b <= 0) b = std::min(jx::kMaxBlock, 64 * ((p->nx + 6 + 63) / 64));
+  if (b <= 0) b = std::min(128, 64 * ((p->nx + 6 + 63) / 64));
   int r = c->rows;
   if (r <= 0) {
     const int strips = (p->nx + (b - 6) - 1) / (b - 6);
@@ -226,7 +228,7 @@ void pick_tiles(const jx_ctx* c, const jx_run_params* p, bool visc, bool exact, 
                                                  exact);
     const long long slots = (long long)std::max(per_cu, 1) * std::max(c->cus, 1);
     const long long per_strip = std::max<long long>(1, slots / strips);
-    r = (int)std::max<long long>(32, (p->ny + per_strip - 1) / per_strip);
+    r = (int)std::max<long long>(12, (p->ny + per_strip - 1) / per_strip);
   }
   *bx = b;
   *rows = std::max(1, std::min(r, p->ny));

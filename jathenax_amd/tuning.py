@@ -78,7 +78,7 @@ def _single_candidates(nx, ny):
     for bx in (256, 128):
         if bx > 64 * ((nx + 6 + 63) // 64) and bx != 128:
             continue
-        for rows in (24, 32, 35, 48, 64, 68, 69, 70, 80, 96):
+        for rows in (8, 10, 12, 16, 24, 32, 35, 48, 64, 68, 69, 70, 80, 96):
             if rows <= ny:
                 out.append((bx, rows))
     return out
