@@ -76,14 +76,14 @@ template <typename T> class Batch : public BatchBase {
     free_[0] = (size_t)pool_bytes_;
     h_inst_.resize(max_inst_);
     slots_.assign(max_inst_, Slot{});
+    done_seen_.assign(max_inst_, 0);
     if (hipStreamSynchronize(stream_) != hipSuccess) return fail("init sync failed");
     return JX_OK;
   }
 
   ~Batch() override {
     if (stream_) hipStreamSynchronize(stream_);
-    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
-    if (graph_) hipGraphDestroy(graph_);
+    drop_graph();
     for (void* p : {(void*)d_inst_, (void*)d_ctrl_, (void*)d_meta_, (void*)d_cnt_,
                     (void*)d_pool_, (void*)d_lists_, (void*)d_flags_})
       if (p) hipFree(p);
@@ -135,6 +135,7 @@ template <typename T> class Batch : public BatchBase {
     Slot& sl = slots_[s];
     sl = Slot{};
     sl.live = true;
+    done_seen_[s] = 0;
     sl.off = (size_t)off;
     sl.bytes = need;
     sl.p = *p;
@@ -238,16 +239,18 @@ template <typename T> class Batch : public BatchBase {
 
   int launch(int rounds) override {
     if (rounds < 0) return fail("run: rounds must be >= 0");
+    const int kind = round_kind();
     if (use_graph_) {
-      if (!graph_exec_) {
-        int rc = capture();
+      if (!graph_exec_[kind]) {
+        int rc = capture(kind);
         if (rc) return rc;
       }
       const int launches = (rounds + graph_rounds_ - 1) / graph_rounds_;
       for (int k = 0; k < launches; ++k)
-        if (hipGraphLaunch(graph_exec_, stream_) != hipSuccess) return fail("graph launch failed");
+        if (hipGraphLaunch(graph_exec_[kind], stream_) != hipSuccess)
+          return fail("graph launch failed");
     } else {
-      for (int k = 0; k < rounds; ++k) launch_round();
+      for (int k = 0; k < rounds; ++k) launch_round(kind);
     }
     if (hipGetLastError() != hipSuccess) return fail("round launch failed");
     return JX_OK;
@@ -269,6 +272,7 @@ template <typename T> class Batch : public BatchBase {
       return fail("poll failed");
     for (int k = 0; k < std::min(cap, max_inst_); ++k) {
       const bool live = slots_[k].live;
+      if (live) done_seen_[k] = h_ctrl_[k].c.done ? 1 : 0;
       if (done) done[k] = live ? (h_ctrl_[k].c.done ? 1 : 0) : -1;
       if (time) time[k] = h_ctrl_[k].c.time;
       if (cycle) cycle[k] = h_ctrl_[k].c.cycle;
@@ -406,36 +410,58 @@ template <typename T> class Batch : public BatchBase {
     return JX_OK;
   }
 
-  void launch_round() {
+  // Which optional launches a round needs: the four kick launches while a forced instance
+  // may still step (kKickRound), the Smagorinsky timestep while one with Smagorinsky may
+  // (kSmagRound).  An instance counts until a poll has seen it done (done is final until
+  // the slot is reused).  Without such instances those launches' work lists are empty, so
+  // leaving them out changes nothing but the round's fixed cost.
+  static constexpr int kKickRound = 1, kSmagRound = 2;
+  int round_kind() const {
+    int kind = 0;
+    for (int k = 0; k < max_inst_; ++k)
+      if (slots_[k].live && !done_seen_[k]) {
+        if (h_inst_[k].forced) kind |= kKickRound;
+        if (h_inst_[k].smag_on) kind |= kSmagRound;
+      }
+    return kind;
+  }
+
+  void launch_round(int kind) {
     const BatchTabs<T> tb = tabs();
     const dim3 g(grid_), blk(kBatchBlock);
     const dim3 gi((max_inst_ + 3) / 4);   // one wave per instance slot
     hipLaunchKernelGGL(batch_control_kernel<T>, dim3(1), dim3(kCtrlBlock), 0, stream_, tb);
     hipLaunchKernelGGL(batch_stage_kernel<T>, g, blk, smem(), stream_, tb);
-    hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 0);
-    hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 0);
-    hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 1);
-    hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 1);
-    hipLaunchKernelGGL(batch_timestep_kernel<T>, g, blk, 0, stream_, tb, -1);
+    if (kind & kKickRound) {
+      hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 0);
+      hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 0);
+      hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 1);
+      hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 1);
+    }
+    if (kind & kSmagRound)
+      hipLaunchKernelGGL(batch_timestep_kernel<T>, g, blk, 0, stream_, tb, -1);
   }
 
-  int capture() {
-    drop_graph();
+  // One graph per round kind, captured on first use.
+  int capture(int kind) {
     if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return fail("graph capture could not start");
-    for (int k = 0; k < graph_rounds_; ++k) launch_round();
-    if (hipStreamEndCapture(stream_, &graph_) != hipSuccess || !graph_)
+    for (int k = 0; k < graph_rounds_; ++k) launch_round(kind);
+    if (hipStreamEndCapture(stream_, &graph_[kind]) != hipSuccess || !graph_[kind])
       return fail("graph capture failed");
-    if (hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0) != hipSuccess)
+    if (hipGraphInstantiate(&graph_exec_[kind], graph_[kind], nullptr, nullptr, 0) !=
+        hipSuccess)
       return fail("graph instantiation failed");
     return JX_OK;
   }
 
   void drop_graph() {
-    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
-    if (graph_) hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
+    for (int k = 0; k < 4; ++k) {
+      if (graph_exec_[k]) hipGraphExecDestroy(graph_exec_[k]);
+      if (graph_[k]) hipGraphDestroy(graph_[k]);
+      graph_exec_[k] = nullptr;
+      graph_[k] = nullptr;
+    }
   }
 
   int device_;
@@ -455,11 +481,12 @@ template <typename T> class Batch : public BatchBase {
   int* h_meta_ = nullptr;
   std::vector<BatchInst<T>> h_inst_;
   std::vector<Slot> slots_;
+  std::vector<char> done_seen_;   // per slot: a poll has seen the instance done
   std::map<size_t, size_t> free_;
   bool use_graph_ = true;
   int graph_rounds_ = 6;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraph_t graph_[4] = {};
+  hipGraphExec_t graph_exec_[4] = {};
 };
 
 }  // namespace jx
