@@ -152,13 +152,19 @@ def configs_leg(steps):
     return out
 
 
-def serving_leg(n_req, rate):
-    """Continuous-batching server: Poisson arrivals, grids 128^2..2048^2, tlim U[0.1, 2]."""
-    from jathenax_amd import workloads
+def serving_leg(n_req, rate, sample=3):
+    """Continuous-batching server: Poisson arrivals, grids 128^2..2048^2, tlim U[0.1, 2].
+    The `sample` smallest served requests are run again in FP64 from the same initial state
+    (the same requests drawn with FP64 forcing tables) on the single-grid path, and their
+    FP32 density error is reported (L1 = mean |rho32 - rho64|)."""
+    import torch
+    from jathenax_amd import run, workloads
     from jathenax_amd.serving import Request, Server
     reqs = [Request(rid=k, spec=spec, arrival=t) for k, (t, spec) in
             enumerate(workloads.serving_requests(n=n_req, rate=rate))]
-    srv = Server(pool_bytes=64 << 30, max_instances=max(8, n_req), keep_results=False)
+    zones = {r.rid: r.spec["U"].shape[0] * r.spec["U"].shape[1] for r in reqs}
+    keep = set(sorted(zones, key=zones.get)[:sample])
+    srv = Server(pool_bytes=64 << 30, max_instances=max(8, n_req), keep_results=True)
     t0 = time.perf_counter()
     out = srv.serve(reqs)
     wall = time.perf_counter() - t0
@@ -166,6 +172,24 @@ def serving_leg(n_req, rate):
     rep = Server.report(out, wall, work)
     rep["workload"] = (f"{n_req} requests, Poisson rate {rate}/s, square grids log-uniform "
                        "128..2048, tlim U[0.1, 2], KH / driven turbulence, PPM4+Roe RK3, FP32")
+    got = {r.rid: r for r in out if r.rid in keep and r.status == "done"}
+    for r in out:
+        if r.rid not in got:
+            r.result = None
+    specs64 = dict(enumerate(s for _, s in workloads.serving_requests(n=n_req, rate=rate,
+                                                                        dtype=np.float64)))
+    errs = []
+    for rid, r in sorted(got.items()):
+        s = specs64[rid]
+        U64, t64, _, c64 = run.evolve_isothermal(
+            torch.tensor(s["U"], dtype=torch.float64, device="cuda"), s["dx"], s["dy"], s["cs"],
+            s["dfloor"], s["integrator"], s["cfl"], s["tlim"], s["nlim"], **s["kw"])
+        rho32 = r.result[0][..., 0].double()
+        errs.append({"rid": rid, "kind": s["kind"], "n": int(s["U"].shape[1]),
+                     "tlim": s["tlim"], "cycles_f32": int(r.result[3]), "cycles_f64": int(c64),
+                     "l1_density": float((rho32 - U64[..., 0]).abs().mean())})
+        r.result = None
+    rep["fp32_vs_fp64"] = errs
     return rep
 
 
