@@ -108,9 +108,28 @@ def ensemble_leg(n_inst):
     work = sum(workloads.zone_stages(s, int(cycles[iid])) for s, iid in zip(specs, ids))
     lat = np.asarray([finish[k] for k in range(len(ids))])
     zones = sum(s["U"].shape[0] * s["U"].shape[1] for s in specs)
+    # Every 32nd instance against its own single-grid run. In FP32 the two kernels agree to
+    # round-off (their compiled contractions differ, and forced runs reduce the kick sums
+    # in another order); in FP64 unforced runs agree bit for bit (tested).
+    from jathenax_amd import run
+    worst = {"unforced": 0.0, "forced": 0.0}
+    same_cycles = 0
+    for k in range(0, len(ids), 32):
+        s = specs[k]
+        U, t, dt, cyc = ens.result(ids[k])
+        Ur, tr, dtr, cr = run.evolve_isothermal(
+            torch.as_tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"], s["dy"],
+            s["cs"], s["dfloor"], s["integrator"], s["cfl"], s["tlim"], s["nlim"], **s["kw"])
+        key = "unforced" if s["kw"].get("forcing") is None else "forced"
+        worst[key] = max(worst[key], float((U - Ur).abs().max() / Ur.abs().max()))
+        same_cycles += int(cyc == cr)
     return {"instances": len(ids), "total_zones": int(zones), "rounds": rounds,
             "wall_s": wall, "p50_s": float(np.percentile(lat, 50)),
             "p99_s": float(np.percentile(lat, 99)), "zone_updates_per_s": work / wall,
+            "checked_vs_single_grid": {"instances": len(range(0, len(ids), 32)),
+                                       "same_cycle_count": same_cycles,
+                                       "max_rel_diff_unforced": worst["unforced"],
+                                       "max_rel_diff_forced": worst["forced"]},
             "workload": "ensemble_mix seed 20261203: sizes log-uniform 64..512 per axis, KH "
                         "(tlim 0.1) / driven turbulence (tlim 0.05), recon plm/ppm4/wenoz, "
                         "rsolver llf/hlle/roe, rk2/rk3, FP32, one batched engine"}

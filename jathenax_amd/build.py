@@ -3,8 +3,11 @@
 Usage: python -m jathenax_amd.build [-j JOBS] [-v]
 
 FP64 translation units are compiled with -ffp-contract=off so that every multiply and
-add rounds as in the reference's IEEE float64 expressions; the FP32 stage units allow
-contraction.  All HIP code is built with packed FP32 instructions disabled.
+add rounds as in the reference's IEEE float64 expressions.  The FP32 stage and batch units
+use -ffp-contract=on: a product is fused only into an add of the same source expression, so
+which operations are fused is fixed by the source text.  With "fast" the backend also fuses
+across statements, and the result then depends on how the row loop is unrolled and so on
+the tile shape.  All HIP code is built with packed FP32 instructions disabled.
 """
 
 import argparse
@@ -44,7 +47,7 @@ def _jobs(n=None):
 def compile_commands():
     cmds = []
     for src in sorted(CSRC.glob("*.hip")):
-        contract = "fast" if src.name.startswith("stage_f32_") else "off"
+        contract = "on" if src.name.startswith(("stage_f32_", "batch_f32")) else "off"
         obj = OBJ / (src.stem + ".o")
         cmds.append((src, obj, [_hipcc(), *HIP_FLAGS, f"-ffp-contract={contract}", "-c",
                                 str(src), "-o", str(obj)]))

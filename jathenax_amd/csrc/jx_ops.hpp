@@ -4,8 +4,10 @@
 // face, in the reference's floating-point expression order.  The float64 translation
 // units are built with -ffp-contract=off, so in FP64 these are the IEEE evaluations of
 // the reference expressions.  In FP32 (Fast<float>) divisions by run constants become
-// reciprocal multiplies and sqrt/rcp use the hardware approximations; FP32 results are
-// checked against the FP64 form by error norms, not bit patterns.
+// reciprocal multiplies and sqrt/rcp use the hardware approximations, and the FP32 stage
+// units fuse a product only into an add of the same expression (-ffp-contract=on), so an
+// FP32 result does not depend on the tile shape; FP32 results are checked against the FP64
+// form by error norms, not bit patterns.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cmath>

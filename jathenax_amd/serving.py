@@ -19,8 +19,9 @@ kernel (the device-resident loop, `big_chunk` steps per server iteration) on a s
 own, while the batch's rounds run on the engine's stream: one grid of that size fills the
 GPU better alone than as tiles among others. At most `max_big` run at once; the rest wait
 without blocking smaller requests. A request's path depends only on its size, so its result
-is the same alone as in any mix. The two paths agree bit for bit on unforced runs and to
-round-off on forced ones (their kick sums are reduced in different orders).
+is the same alone as in any mix. In FP32 the two paths agree to round-off (the kernels are
+separate code that may fuse different products into adds, and forced runs reduce their kick
+sums in different orders).
 
 Failure handling: a request whose time or state is non-finite, or that exceeds its round
 budget, is evicted with an error status instead of blocking the batch.

@@ -89,6 +89,30 @@ def test_tile_shapes_are_bitwise_equal(jx, bx, rows):
     np.testing.assert_array_equal(got, base)
 
 
+@pytest.mark.parametrize("recon,solver", [("plm", "hlle"), ("ppm4", "roe"), ("ppmx", "llf"),
+                                          ("wenoz", "roe")])
+def test_fp32_tile_shapes_are_bitwise_equal(jx, recon, solver):
+    """FP32 fuses products into adds only within one source expression, so the unrolled row
+    loop computes every row with the same operations wherever a tile starts."""
+    from jathenax_amd.device import context_for
+    rng = np.random.default_rng(5)
+    U = np.empty((97, 131, 4))
+    U[..., 0] = 1.0 + 0.3 * rng.random((97, 131))
+    U[..., 1:] = 0.3 * rng.standard_normal((97, 131, 3))
+    Ud = torch.tensor(U, dtype=torch.float32, device="cuda")
+    ctx = context_for(Ud)
+    kw = dict(viscosity=1e-3, smagorinsky_cs=0.1, reconstruct=recon, rsolver=solver)
+    ctx.set_tiles(0, 0)
+    base = _np(jx.isothermal_step(Ud, 1e-3, 0.01, 0.01, 1.0, 1e-10, "rk3", **kw))
+    try:
+        for bx, rows in ((64, 1), (64, 7), (128, 13), (128, 16), (192, 5), (256, 64)):
+            ctx.set_tiles(bx, rows)
+            got = _np(jx.isothermal_step(Ud, 1e-3, 0.01, 0.01, 1.0, 1e-10, "rk3", **kw))
+            np.testing.assert_array_equal(got, base, err_msg=str((bx, rows)))
+    finally:
+        ctx.set_tiles(0, 0)
+
+
 @pytest.mark.parametrize("tlim,nlim,tag", [(0.2, 100000, "tlim"), (10.0, 100, "n100")])
 def test_smooth_run_100_cycles(jx, ref_vectors, tlim, nlim, tag):
     U0 = ref_vectors["run_smooth_U0"]
