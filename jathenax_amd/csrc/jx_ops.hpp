@@ -135,6 +135,51 @@ __device__ __forceinline__ T ppmx_limit_edge(T a, T b, T e, T ca, T cb) {
   return ((a - e) * (b - e) > T(0)) ? corr : e;
 }
 
+// FP32 WENO-Z (fluxes.py:87-117) from first differences.  With dm2 = qm1 - qm2, dm = q - qm1,
+// dp = qp1 - q, dp2 = qp2 - qp1 the smoothness indicators are
+//   4 b_k = (13/3) c_k^2 + s_k^2,  c = (dm - dm2, dp - dm, dp2 - dp),
+//                                  s = (3 dm - dm2, dm + dp, dp2 - 3 dp),
+// and the weights depend only on ratios of the b_k + eps, so 4 b_k + 4 eps are used as they
+// are.  The three candidate values at the high edge differ from the central one,
+// p1 = q + (dm + 2 dp) / 6, by third differences: p0 - p1 = ta / 3 and p2 - p1 = tb / 6 with
+// ta = c0 - c1, tb = c1 - c2 (at the low edge, p1' = q - (2 dm + dp) / 6, p0' - p1' = -tb / 3
+// and p2' - p1' = -ta / 6).  With the linear weights (0.1, 0.6, 0.3) folded in:
+//   hi = p1  + (w0 ta / 3 + w2 tb / 2) / (w0 + 6 w1 + 3 w2),
+//   lo = p1' - (w2 tb / 3 + w0 ta / 2) / (w2 + 6 w1 + 3 w0).
+// Same weights and edge values as the reference expressions, in about two thirds of the
+// operations, and without their cancellation between large stencil sums: the float32
+// error against the float64 reference form is about half that of the reference form in
+// float32.  The weight sums are at least 1 (the smallest d_k gives w = b^2 + 1 when
+// tau >= dmin, and base = 1 otherwise), so both reciprocals are finite.
+__device__ __forceinline__ void wenoz_edges_fast(float qm2, float qm1, float q, float qp1,
+                                                 float qp2, float eps, float& lo, float& hi) {
+  const float dm2 = qm1 - qm2, dm = q - qm1, dp = qp1 - q, dp2 = qp2 - qp1;
+  const float c0 = dm - dm2, c1 = dp - dm, c2 = dp2 - dp;
+  const float s0 = fmaf(3.0f, dm, -dm2), s1 = dm + dp, s2 = fmaf(-3.0f, dp, dp2);
+  const float k = 13.0f / 3.0f;
+  const float b0 = fmaf(k * c0, c0, s0 * s0);
+  const float b1 = fmaf(k * c1, c1, s1 * s1);
+  const float b2 = fmaf(k * c2, c2, s2 * s2);
+  const float e4 = 4.0f * eps;
+  const float tau = fabsf(b0 - b2);
+  const float d0 = b0 + e4, d1 = b1 + e4, d2 = b2 + e4;
+  const float dmin = fminf(d0, fminf(d1, d2));
+  const float b = dmin * __builtin_amdgcn_rcpf(fmaxf(tau, dmin));   // dmin / scale
+  const float g = tau * b;                                           // tau dmin / scale
+  const float base = b * b;
+  const float t0 = g * __builtin_amdgcn_rcpf(d0), t1 = g * __builtin_amdgcn_rcpf(d1),
+              t2 = g * __builtin_amdgcn_rcpf(d2);
+  const float w0 = fmaf(t0, t0, base), w1 = fmaf(t1, t1, base), w2 = fmaf(t2, t2, base);
+  const float ta = c0 - c1, tb = c1 - c2;
+  const float u = w0 * ta, v = w2 * tb;
+  hi = fmaf(fmaf(u, 1.0f / 3.0f, 0.5f * v),
+            __builtin_amdgcn_rcpf(fmaf(3.0f, w2, fmaf(6.0f, w1, w0))),
+            fmaf(1.0f / 6.0f, fmaf(2.0f, dp, dm), q));
+  lo = fmaf(-fmaf(v, 1.0f / 3.0f, 0.5f * u),
+            __builtin_amdgcn_rcpf(fmaf(3.0f, w0, fmaf(6.0f, w1, w2))),
+            fmaf(-1.0f / 6.0f, fmaf(2.0f, dm, dp), q));
+}
+
 template <int RECON, typename T>
 __device__ __forceinline__ void cell_edges(T qm2, T qm1, T q, T qp1, T qp2, T eps, T& lo,
                                            T& hi) {
@@ -179,6 +224,10 @@ __device__ __forceinline__ void cell_edges(T qm2, T qm1, T q, T qp1, T qp2, T ep
     hi = ext ? q + dh * ratio : hm;
   } else {
     // WENO-Z, fluxes.py:87-117
+    if constexpr (Fast<T>::value) {
+      wenoz_edges_fast(qm2, qm1, q, qp1, qp2, eps, lo, hi);
+      return;
+    }
     const T c13 = T(13.0 / 12.0);
     const T b0 = c13 * sqr((qm2 + q) - T(2) * qm1) + T(0.25) * sqr((qm2 + T(3) * q) - T(4) * qm1);
     const T b1 = c13 * sqr((qm1 + qp1) - T(2) * q) + T(0.25) * sqr(qm1 - qp1);
