@@ -248,7 +248,7 @@ int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap);
 int jx_batch_result(jx_batch* b, int id, void* U_out, double* time, double* dt, int* cycle);
 int jx_batch_state(jx_batch* b, int id, void** state_out);
 int jx_batch_set_graph(jx_batch* b, int use_graph, int rounds_per_graph);
-/* Tiles per resident wave that each round's tile heights aim for (default 1); 0 keeps
+/* Tiles per resident wave that each round's tile heights aim for (default 2); 0 keeps
  * every instance at its own cap.  Results do not depend on it. */
 int jx_batch_set_fill(jx_batch* b, int fill);
 

@@ -469,7 +469,7 @@ template <typename T> class Batch : public BatchBase {
   long long pool_bytes_;
   hipStream_t stream_ = nullptr;
   int cus_ = 0, grid_ = 1, nwaves_ = 4;
-  int fill_ = 1;   // tiles per resident wave that the band heights aim for
+  int fill_ = 2;   // tiles per resident wave that the band heights aim for
   BatchInst<T>* d_inst_ = nullptr;
   BatchCtrl* d_ctrl_ = nullptr;
   int* d_meta_ = nullptr;

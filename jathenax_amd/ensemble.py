@@ -64,7 +64,7 @@ class Ensemble:
         self._check(_lib.load().jx_batch_set_graph(self.handle, int(bool(on)), int(rounds)),
                     "set_graph")
 
-    def set_fill(self, fill=1):
+    def set_fill(self, fill=2):
         """Tiles per resident wave that each round's tile heights aim for; 0 keeps every
         instance at its own cap.  Results do not depend on it."""
         self._check(_lib.load().jx_batch_set_fill(self.handle, int(fill)), "set_fill")
