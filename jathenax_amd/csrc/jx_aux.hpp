@@ -419,6 +419,225 @@ __global__ void __launch_bounds__(kAuxBlock) render_force_kernel(RenderArgs<T> a
                    on ? i : 0, on, yb, min(8, a.ny - yb), lds);
 }
 
+// ---------------------------------------------------------------------------------------
+// FP32 render by x-wavenumber groups.  Each mode term of forcing.py:115-122 is
+//   re a_c0 - im a_c1 = cx (cy a_c0 - sy a_c1) - sx (sy a_c0 + cy a_c1),
+// so modes whose x basis rows are equal (cx, sx), or equal up to the sign of sx (the -kx
+// partner of kx), share one x factor:
+//   f_c(i, j) = sum_g cx_g(i) P_gc(j) - sx_g(i) Q_gc(j),
+//   P_gc = sum_{m in g} cy_m a_mc0 - sy_m a_mc1,  Q_gc = sum_{m in g} s_m (sy_m a_mc0 + cy_m a_mc1)
+// with s_m = +-1 the sign of the mode's sx against its group's.  The per-row sums P, Q cost
+// a few operations per row and mode; each cell then costs 4 multiply-adds per group instead
+// of 8 per mode (the Cartesian band 1 <= |k| <= 3 has 28 modes in 4 groups).  Groups are
+// found from the tables themselves (exact comparison), so any basis is rendered correctly;
+// one with no shared rows has one group per mode.  FP64 keeps the per-mode loop above, whose
+// sum order is the reference's.
+//
+// Group table (ints), for M modes: [0] group count G, [1 .. G+1] first member of each group
+// in `order` (G+1 entries), then order[M] (modes by group; the first member's tables are the
+// group's), sign[M] (+1 / -1 per mode), and scratch gid[M], rep[M], rel[M*M].
+// ---------------------------------------------------------------------------------------
+
+// Offset of rel[] in the group table: G, gstart[M+1], order, sign, gid, rep.
+__host__ __device__ inline size_t mode_rel_offset(int M) { return 2 + 5 * (size_t)M; }
+__host__ __device__ inline size_t mode_group_ints(int M) {
+  return mode_rel_offset(M) + (size_t)M * M;
+}
+
+// rel[m * M + n] for n < m: 1 when modes m and n have equal x rows, 2 when cx is equal and sx
+// negated, 0 otherwise.  One block per (m, n) pair.
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) mode_relation_kernel(const T* cx, const T* sx,
+                                                                  int nx, int M, int* grp) {
+  const int m = blockIdx.y, n = blockIdx.x;
+  if (n >= m) return;
+  int same = 1, anti = 1;
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+    const T c0 = cx[(size_t)m * nx + i], c1 = cx[(size_t)n * nx + i];
+    const T s0 = sx[(size_t)m * nx + i], s1 = sx[(size_t)n * nx + i];
+    same &= (c0 == c1) && (s0 == s1);
+    anti &= (c0 == c1) && (s0 == -s1);
+  }
+  same = __syncthreads_and(same);
+  anti = __syncthreads_and(anti);
+  int* rel = grp + mode_rel_offset(M);
+  if (threadIdx.x == 0) rel[(size_t)m * M + n] = same ? 1 : (anti ? 2 : 0);
+}
+
+// One thread: each mode joins the first earlier group whose first member it matches.
+static __global__ void mode_groups_kernel(int M, int* grp) {
+  int* gstart = grp + 1;
+  int* order = gstart + (M + 1);
+  int* sign = order + M;
+  int* gid = sign + M;
+  int* rep = gid + M;
+  const int* rel = grp + mode_rel_offset(M);
+  int G = 0;
+  for (int m = 0; m < M; ++m) {
+    int g = -1, s = 1;
+    for (int h = 0; h < G && g < 0; ++h) {
+      const int r = rel[(size_t)m * M + rep[h]];
+      if (r) {
+        g = h;
+        s = (r == 1) ? 1 : -1;
+      }
+    }
+    if (g < 0) {
+      g = G++;
+      rep[g] = m;
+    }
+    gid[m] = g;
+    sign[m] = s;
+  }
+  for (int g = 0; g <= G; ++g) gstart[g] = 0;
+  for (int m = 0; m < M; ++m) ++gstart[gid[m] + 1];
+  for (int g = 0; g < G; ++g) gstart[g + 1] += gstart[g];
+  for (int g = 0; g < G; ++g) rep[g] = gstart[g];   // fill cursor
+  for (int m = 0; m < M; ++m) order[rep[gid[m]]++] = m;
+  grp[0] = G;
+}
+
+constexpr int kGroupChunk = 32;
+
+// Blocks as render_force_kernel: kAuxBlock columns by render_rows_per_block rows.
+static __global__ void __launch_bounds__(kAuxBlock) render_grouped_kernel(RenderArgs<float> a,
+                                                                   const int* grp) {
+  __shared__ float4 pq[kRenderRows][kGroupChunk];   // (P_g0, Q_g0, P_g1, Q_g1)
+  int row = a.row;
+  if (a.ctrl) {
+    if (a.ctrl->done || !a.ctrl->render) return;
+    row = a.ctrl->update;
+  }
+  const int M = a.n_modes, nx = a.nx, ny = a.ny;
+  const bool fill = (row < 0) || (row >= a.n_updates);
+  const float* amp = a.table + (size_t)(fill ? 0 : row) * M * 4;
+  const float nan = __builtin_nanf("");
+  const int G = grp[0];
+  const int* gstart = grp + 1;
+  const int* order = gstart + (M + 1);
+  const int* sign = order + M;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = i < nx;
+  const int rb = render_rows_per_block(nx, ny);
+  const int y0 = blockIdx.y * rb;
+  const int nr = min(rb, ny - y0);
+  float f0[kRenderRows], f1[kRenderRows];
+#pragma unroll
+  for (int r = 0; r < kRenderRows; ++r) f0[r] = f1[r] = 0.0f;
+  for (int g0 = 0; g0 < G; g0 += kGroupChunk) {
+    const int gc = min(kGroupChunk, G - g0);
+    __syncthreads();   // the previous chunk's sums are consumed
+    for (int e = threadIdx.x; e < gc * rb; e += blockDim.x) {
+      const int q = e / rb, r = e - q * rb;
+      const int j = y0 + min(r, nr - 1);
+      float p0 = 0.0f, q0 = 0.0f, p1 = 0.0f, q1 = 0.0f;
+      for (int k = gstart[g0 + q]; k < gstart[g0 + q + 1]; ++k) {
+        const int m = order[k];
+        const float s = (float)sign[m];
+        const float cyv = a.cy[(size_t)m * ny + j], syv = a.sy[(size_t)m * ny + j];
+        const float a00 = fill ? nan : amp[m * 4 + 0], a01 = fill ? nan : amp[m * 4 + 1];
+        const float a10 = fill ? nan : amp[m * 4 + 2], a11 = fill ? nan : amp[m * 4 + 3];
+        p0 = fmaf(cyv, a00, fmaf(-syv, a01, p0));
+        q0 = fmaf(s, fmaf(syv, a00, cyv * a01), q0);
+        p1 = fmaf(cyv, a10, fmaf(-syv, a11, p1));
+        q1 = fmaf(s, fmaf(syv, a10, cyv * a11), q1);
+      }
+      pq[r][q] = make_float4(p0, q0, p1, q1);
+    }
+    __syncthreads();
+    for (int q = 0; q < gc; ++q) {
+      const int m = order[gstart[g0 + q]];
+      const float cxv = on ? a.cx[(size_t)m * nx + i] : 0.0f;
+      const float sxv = on ? a.sx[(size_t)m * nx + i] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < kRenderRows; ++r) {
+        if (r < nr) {
+          const float4 v = pq[r][q];
+          f0[r] = fmaf(cxv, v.x, fmaf(-sxv, v.y, f0[r]));
+          f1[r] = fmaf(cxv, v.z, fmaf(-sxv, v.w, f1[r]));
+        }
+      }
+    }
+  }
+  if (on) {
+    gmem_t<float>* raw = gmem(a.raw);
+#pragma unroll
+    for (int r = 0; r < kRenderRows; ++r) {
+      if (r < nr) {
+        raw[((size_t)(y0 + r) * nx + i) * 2 + 0] = f0[r];
+        raw[((size_t)(y0 + r) * nx + i) * 2 + 1] = f1[r];
+      }
+    }
+  }
+}
+
+// The grouped render for one wave's 64 columns over rows y0 .. y0+nr-1 (nr <= 8), as
+// render_wave: the per-row sums of up to kWaveGroupChunk groups go to the wave's LDS slice
+// `lds` (16-byte aligned, 4 * kWaveGroupChunk * 8 floats), then each lane adds its column.
+constexpr int kWaveGroupChunk = 16;
+
+__device__ __forceinline__ void render_wave_grouped(gmem_t<float>* raw, const float* amp,
+                                                    bool fill, const float* cx,
+                                                    const float* sx, const float* cy,
+                                                    const float* sy, const int* grp, int nx,
+                                                    int ny, int M, int i, bool on, int y0,
+                                                    int nr, float* lds) {
+  const int lane = threadIdx.x & 63;
+  const float nan = __builtin_nanf("");
+  const int G = grp[0];
+  const int* gstart = grp + 1;
+  const int* order = gstart + (M + 1);
+  const int* sign = order + M;
+  float f0[8], f1[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) f0[r] = f1[r] = 0.0f;
+  float4* pq = reinterpret_cast<float4*>(lds);   // [group][row]
+  for (int g0 = 0; g0 < G; g0 += kWaveGroupChunk) {
+    const int gc = min(kWaveGroupChunk, G - g0);
+    render_wave_sync();   // the previous chunk's sums are consumed
+    for (int e = lane; e < gc * 8; e += 64) {
+      const int q = e >> 3, r = e & 7;
+      const int j = y0 + (r < nr ? r : 0);
+      float p0 = 0.0f, q0 = 0.0f, p1 = 0.0f, q1 = 0.0f;
+      for (int k = gstart[g0 + q]; k < gstart[g0 + q + 1]; ++k) {
+        const int m = order[k];
+        const float sg = (float)sign[m];
+        const float cyv = cy[(size_t)m * ny + j], syv = sy[(size_t)m * ny + j];
+        const float a00 = fill ? nan : amp[m * 4 + 0], a01 = fill ? nan : amp[m * 4 + 1];
+        const float a10 = fill ? nan : amp[m * 4 + 2], a11 = fill ? nan : amp[m * 4 + 3];
+        p0 = fmaf(cyv, a00, fmaf(-syv, a01, p0));
+        q0 = fmaf(sg, fmaf(syv, a00, cyv * a01), q0);
+        p1 = fmaf(cyv, a10, fmaf(-syv, a11, p1));
+        q1 = fmaf(sg, fmaf(syv, a10, cyv * a11), q1);
+      }
+      pq[e] = make_float4(p0, q0, p1, q1);
+    }
+    render_wave_sync();
+    for (int q = 0; q < gc; ++q) {
+      const int m = order[gstart[g0 + q]];
+      const float cxv = on ? cx[(size_t)m * nx + i] : 0.0f;
+      const float sxv = on ? sx[(size_t)m * nx + i] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        if (r < nr) {
+          const float4 v = pq[q * 8 + r];
+          f0[r] = fmaf(cxv, v.x, fmaf(-sxv, v.y, f0[r]));
+          f1[r] = fmaf(cxv, v.z, fmaf(-sxv, v.w, f1[r]));
+        }
+      }
+    }
+  }
+  if (on) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r < nr) {
+        raw[((size_t)(y0 + r) * nx + i) * 2 + 0] = f0[r];
+        raw[((size_t)(y0 + r) * nx + i) * 2 + 1] = f1[r];
+      }
+    }
+  }
+}
+
 template <typename T> struct KickArgs {
   T* u;                 // post-drag state (ny, nx, nv), updated in place by the kick
   const T* raw;

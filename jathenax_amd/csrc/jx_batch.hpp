@@ -42,6 +42,7 @@ template <typename T> struct BatchInst {
   T* raw;                // rendered raw force (forced instances)
   double* part;          // kick sums, 3 doubles per (strip, 8-row group), strip-major
   const T* coef; const T* cx; const T* sx; const T* cy; const T* sy;
+  const int* grp;        // FP32 render: mode groups of the basis (nullptr: per-mode render)
   int nx, ny, nv, recon, rs, nstages;
   int visc_on, smag_on, drag_on, forced, n_updates, n_modes;
   long long reg_stride;  // elements from A to B, B to C, C to S
@@ -566,6 +567,7 @@ __device__ __forceinline__ void batch_stage_dispatch(const BatchInst<T>& I, Batc
 
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock, 2) batch_stage_kernel(BatchTabs<T> b) {
+  static_assert(kBatchSlice * 64 >= 4 * kWaveGroupChunk * 8, "render slice too small");
   extern __shared__ double bsm[];
   T* lds = reinterpret_cast<T*>(bsm) + (threadIdx.x >> 6) * kBatchSlice * 64;
   const int* L = b.lists + WL_STAGE * list_stride(b.cap);
@@ -699,6 +701,14 @@ __device__ __forceinline__ void batch_render_tile(const BatchInst<T>& I, const B
   const T* amp = I.coef + (size_t)(fill ? 0 : row) * I.n_modes * 4;
   const int i = tl.x0 + t;
   const bool on = t < kTileCols && i < I.nx;   // per lane; the row loop stays uniform
+  if constexpr (Fast<T>::value) {
+    if (I.grp) {
+      for (int yb = tl.y0; yb < tl.y0 + tl.rows; yb += 8)
+        render_wave_grouped(gmem(I.raw), amp, fill, I.cx, I.sx, I.cy, I.sy, I.grp, I.nx, I.ny,
+                            I.n_modes, on ? i : 0, on, yb, min(8, tl.y0 + tl.rows - yb), lds);
+      return;
+    }
+  }
   for (int yb = tl.y0; yb < tl.y0 + tl.rows; yb += 8)
     render_wave<T>(gmem(I.raw), amp, fill, I.cx, I.sx, I.cy, I.sy, I.nx, I.ny, I.n_modes,
                    on ? i : 0, on, yb, min(8, tl.y0 + tl.rows - yb), lds);
@@ -976,7 +986,8 @@ __global__ void batch_resume_kernel(BatchTabs<T> b, int slot, double time, doubl
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_render_slot_kernel(BatchTabs<T> b,
                                                                         int slot) {
-  __shared__ T slices[kBatchBlock / 64][2 * kRenderModes * 8];
+  static_assert(2 * kRenderModes * 8 >= 4 * kWaveGroupChunk * 8, "render slice too small");
+  __shared__ __align__(16) T slices[kBatchBlock / 64][2 * kRenderModes * 8];
   const BatchInst<T> I = load_const(b.inst + slot);
   const BatchCtrl& C = b.ctrl[slot];
   const int rows = I.rows_cap;

@@ -64,6 +64,10 @@ template <typename T> class Batch : public BatchBase {
         hipMalloc(&d_lists_, sizeof(int) * WL_TOTAL * list_stride(max_inst_)) != hipSuccess ||
         hipMalloc(&d_flags_, sizeof(int) * max_inst_) != hipSuccess ||
         hipMalloc(&d_pool_, pool_bytes_) != hipSuccess ||
+        (Fast<T>::value &&
+         (hipMalloc(&d_groups_, sizeof(int) * kGroupTable * max_inst_) != hipSuccess ||
+          hipMalloc(&d_gscratch_, sizeof(int) * mode_group_ints(kMaxGroupModes)) !=
+              hipSuccess)) ||
         hipHostMalloc(&h_ctrl_, sizeof(BatchCtrl) * max_inst_, hipHostMallocDefault) !=
             hipSuccess ||
         hipHostMalloc(&h_meta_, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess)
@@ -85,7 +89,8 @@ template <typename T> class Batch : public BatchBase {
     if (stream_) hipStreamSynchronize(stream_);
     drop_graph();
     for (void* p : {(void*)d_inst_, (void*)d_ctrl_, (void*)d_meta_, (void*)d_cnt_,
-                    (void*)d_pool_, (void*)d_lists_, (void*)d_flags_})
+                    (void*)d_pool_, (void*)d_lists_, (void*)d_flags_, (void*)d_groups_,
+                    (void*)d_gscratch_})
       if (p) hipFree(p);
     if (h_ctrl_) hipHostFree(h_ctrl_);
     if (h_meta_) hipHostFree(h_meta_);
@@ -158,6 +163,21 @@ template <typename T> class Batch : public BatchBase {
       I.sx = static_cast<const T*>(f->sin_x);
       I.cy = static_cast<const T*>(f->cos_y);
       I.sy = static_cast<const T*>(f->sin_y);
+      if (Fast<T>::value && f->n_modes <= kMaxGroupModes) {
+        // Mode groups of the x basis (render_grouped_kernel): found in the shared scratch,
+        // then the part the render reads (G, gstart, order, sign) is kept per slot.  Adds
+        // are ordered on the batch stream, so one scratch serves them all.
+        const int M = f->n_modes;
+        if (M > 1)
+          hipLaunchKernelGGL(mode_relation_kernel<T>, dim3(M, M), dim3(kAuxBlock), 0, stream_,
+                             I.cx, I.sx, p->nx, M, d_gscratch_);
+        hipLaunchKernelGGL(mode_groups_kernel, dim3(1), dim3(1), 0, stream_, M, d_gscratch_);
+        int* tab = d_groups_ + (size_t)s * kGroupTable;
+        if (hipMemcpyAsync(tab, d_gscratch_, sizeof(int) * (2 + 3 * (size_t)M),
+                           hipMemcpyDeviceToDevice, stream_) != hipSuccess)
+          return fail("add: mode group copy failed");
+        I.grp = tab;
+      }
       I.n_updates = f->n_updates;
       I.n_modes = f->n_modes;
     }
@@ -477,6 +497,12 @@ template <typename T> class Batch : public BatchBase {
   int* d_lists_ = nullptr;
   int* d_flags_ = nullptr;
   void* d_pool_ = nullptr;
+  // FP32 forcing render: per-slot mode-group tables (bases of up to kMaxGroupModes modes)
+  // and the scratch they are computed in.
+  static constexpr int kMaxGroupModes = 256;
+  static constexpr int kGroupTable = 2 + 3 * kMaxGroupModes;
+  int* d_groups_ = nullptr;
+  int* d_gscratch_ = nullptr;
   BatchCtrl* h_ctrl_ = nullptr;
   int* h_meta_ = nullptr;
   std::vector<BatchInst<T>> h_inst_;

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -54,6 +55,8 @@ struct jx_ctx {
   size_t buf_bytes = 0;
   void* raw = nullptr;        // rendered raw force (evolve with forcing)
   size_t raw_bytes = 0;
+  void* groups = nullptr;     // FP32 render: mode groups of the forcing basis
+  size_t groups_bytes = 0;
   double* partial = nullptr;  // per-block partial sums
   size_t partial_count = 0;
   double* sums = nullptr;     // small device result vector
@@ -118,6 +121,34 @@ int ensure_buffers(jx_ctx* c, size_t bytes) {
 dim3 render_grid(int nx, int ny) {
   const int rb = jx::render_rows_per_block(nx, ny);
   return dim3((unsigned)((nx + jx::kAuxBlock - 1) / jx::kAuxBlock), (unsigned)((ny + rb - 1) / rb));
+}
+
+// FP32 renders sum modes by groups that share their x basis rows (render_grouped_kernel);
+// the groups are found from the basis tables on the context stream before the first render.
+int prepare_groups(jx_ctx* c, const float* cx, const float* sx, int nx, int n_modes) {
+  int rc = ensure(c, &c->groups, &c->groups_bytes, jx::mode_group_ints(n_modes) * sizeof(int));
+  if (rc) return rc;
+  int* grp = static_cast<int*>(c->groups);
+  if (n_modes > 1)
+    hipLaunchKernelGGL(jx::mode_relation_kernel<float>, dim3(n_modes, n_modes),
+                       dim3(jx::kAuxBlock), 0, c->stream, cx, sx, nx, n_modes, grp);
+  hipLaunchKernelGGL(jx::mode_groups_kernel, dim3(1), dim3(1), 0, c->stream, n_modes, grp);
+  return JX_OK;
+}
+
+template <typename T> int prepare_render(jx_ctx* c, const jx::RenderArgs<T>& ra) {
+  if constexpr (std::is_same<T, float>::value)
+    return prepare_groups(c, ra.cx, ra.sx, ra.nx, ra.n_modes);
+  return JX_OK;
+}
+
+template <typename T> void launch_render(jx_ctx* c, const jx::RenderArgs<T>& ra) {
+  if constexpr (std::is_same<T, float>::value)
+    hipLaunchKernelGGL(jx::render_grouped_kernel, render_grid(ra.nx, ra.ny), dim3(jx::kAuxBlock),
+                       0, c->stream, ra, static_cast<const int*>(c->groups));
+  else
+    hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(ra.nx, ra.ny),
+                       dim3(jx::kAuxBlock), 0, c->stream, ra);
 }
 
 dim3 aux_grid(size_t n) {
@@ -500,14 +531,15 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     ra.n_modes = f->n_modes;
     ra.n_updates = f->n_updates;
     ra.ctrl = c->ctrl;
+    rc = prepare_render<T>(c, ra);
+    if (rc) return rc;
     JX_HIP(c, hipMemsetAsync(c->raw, 0, n * 2 * sizeof(T), c->stream));
     if (s0.update >= 0) {
       // Resume: the raw force of the OU row in effect (forcing.py:111-122).
       jx::RenderArgs<T> r0 = ra;
       r0.ctrl = nullptr;
       r0.row = s0.update;
-      hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(p->nx, p->ny),
-                         dim3(jx::kAuxBlock), 0, c->stream, r0);
+      launch_render<T>(c, r0);
     }
   }
   const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
@@ -516,9 +548,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
                      c->stream, timestep_args<T>(c, p, in, 0, c->ctrl));
   auto one_step = [&](bool first) -> int {
     hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
-    if (forced)
-      hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(p->nx, p->ny),
-                         dim3(jx::kAuxBlock), 0, c->stream, ra);
+    if (forced) launch_render<T>(c, ra);
     int r = launch_step<T>(c, s, in, out, first, raw, !smag, hook);
     if (r) return r;
     if (smag)
@@ -551,9 +581,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     // steps past the end must not call it.
     chunk = hook ? 1 : std::max<long long>(1, std::min<long long>(want, 512));
     {
-      if (forced)
-        hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(p->nx, p->ny),
-                           dim3(jx::kAuxBlock), 0, c->stream, ra);
+      if (forced) launch_render<T>(c, ra);
       int r = launch_step<T>(c, s, in, out, false, raw, !smag, hook);
       if (r) return r;
       if (smag)
@@ -716,6 +744,7 @@ void jx_ctx_destroy(jx_ctx* c) {
   for (auto& b : c->buf)
     if (b) (void)hipFree(b);
   if (c->raw) (void)hipFree(c->raw);
+  if (c->groups) (void)hipFree(c->groups);
   if (c->partial) (void)hipFree(c->partial);
   if (c->ctrl) (void)hipFree(c->ctrl);
   if (c->sums) (void)hipFree(c->sums);
@@ -944,7 +973,9 @@ int jx_render_force(jx_ctx* c, int dtype, int ny, int nx, int n_modes, const voi
                             static_cast<const float*>(cos_x), static_cast<const float*>(sin_x),
                             static_cast<const float*>(cos_y), static_cast<const float*>(sin_y),
                             nx, ny, n_modes, 1, nullptr, 0};
-    hipLaunchKernelGGL(jx::render_force_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream, a);
+    int rc = prepare_render<float>(c, a);
+    if (rc) return rc;
+    launch_render<float>(c, a);
   }
   JX_HIP(c, hipGetLastError());
   return JX_OK;

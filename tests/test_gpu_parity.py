@@ -207,6 +207,41 @@ def test_forcing_operators(jx, ref_vectors):
         assert np.max(np.abs(_np(acc) - want)) <= 1e-13 * max(np.max(np.abs(want)), 1.0)
 
 
+def test_fp32_render_by_mode_groups(jx, ref_vectors):
+    """FP32 renders sum modes in groups that share their x basis rows (equal, or with sx
+    negated).  The Cartesian band (28 modes, 4 groups) must give the reference render to
+    float32 rounding, and so must bases where no rows, or only some, are shared."""
+    from jathenax_amd import forcing as jf
+    modes = jf.cartesian_modes(1, 3, (-3, 3), (-3, 3))
+    coef = jf.modal_coefficients(modes, (1.0, 1.0), 1, 3, 2.0, 0.7, 99, 0.0, 0.01, 3)
+    x = ((np.arange(16) + 0.5) / 16 - 0.5).astype(np.float32)
+    y = ((np.arange(12) + 0.5) / 12 - 0.5).astype(np.float32)
+    basis = jf.forcing_basis(modes, x, y, (1.0, 1.0))
+    raw = _np(jf.render_force(torch.tensor(coef[1], dtype=torch.float32, device="cuda"), basis))
+    want = ref_vectors["render_force"]
+    assert np.max(np.abs(raw - want)) <= 1e-5 * np.max(np.abs(want))
+
+    rng = np.random.default_rng(3)
+    M, nx, ny = 9, 300, 37
+    cx = rng.uniform(-1, 1, (M, nx)).astype(np.float32)
+    sx = rng.uniform(-1, 1, (M, nx)).astype(np.float32)
+    cy = rng.uniform(-1, 1, (M, ny)).astype(np.float32)
+    sy = rng.uniform(-1, 1, (M, ny)).astype(np.float32)
+    amp = rng.uniform(-1, 1, (M, 2, 2)).astype(np.float32)
+    for shared in ((), ((0, 3), (0, 5), (2, 8)), ((1, 4),)):
+        cxs, sxs = cx.copy(), sx.copy()
+        for a, b in shared:            # (0,3), (0,5), (2,8): equal rows; (1,4): sx negated
+            cxs[b] = cxs[a]
+            sxs[b] = -sxs[a] if (a, b) == (1, 4) else sxs[a]
+        want = ox.render_force(amp.astype(np.float64),
+                               tuple(t.astype(np.float64) for t in (cxs, sxs, cy, sy)))
+        got = _np(jf.render_force(torch.tensor(amp, device="cuda"),
+                                  tuple(torch.tensor(t, device="cuda")
+                                        for t in (cxs, sxs, cy, sy))))
+        # 8 products per mode, each within |amp|: float32 accumulation error bound.
+        assert np.max(np.abs(got - want)) <= 8 * M * 2.0 ** -23 * 2.0, shared
+
+
 def test_fp32_tracks_fp64(jx):
     from jathenax_amd import problems
     U0 = problems.smooth_wave(128, 128)
