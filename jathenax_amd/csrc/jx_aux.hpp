@@ -667,42 +667,58 @@ __global__ void __launch_bounds__(kAuxBlock) kick_first_moments_kernel(KickArgs<
 }
 
 // Sum per-block partials in a fixed order.  mode 0: mean force mu = sum(rho r)/sum(rho)
-// (forcing.py:141).  mode 1: m0, m1 and the kick scale (forcing.py:144-148).
+// (forcing.py:141).  mode 1: m0, m1 and the kick scale (forcing.py:144-148).  mode 2 (the
+// FP32 single pass, 7 sums per block: rho, rho r, rho |r|^2, m.r, m): mu, then
+//   sum rho |r - mu|^2 = sum rho |r|^2 - mu . sum rho r,   sum m.(r - mu) = sum m.r - mu . sum m,
+// and the scale from them.
 static __global__ void reduce_kick_kernel(Ctrl* c, const double* partial, int nblocks, int mode,
                                    double ncells) {
   if (c->done) return;
-  __shared__ double s[3][256];
-  double a0 = 0, a1 = 0, a2 = 0;
+  constexpr int K = 7;
+  __shared__ double s[K][256];
+  const int nk = mode == 2 ? 7 : 3;
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-    a0 += partial[3 * b + 0];
-    a1 += partial[3 * b + 1];
-    a2 += partial[3 * b + 2];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (k < nk) a[k] += partial[(size_t)nk * b + k];
   }
-  s[0][threadIdx.x] = a0;
-  s[1][threadIdx.x] = a1;
-  s[2][threadIdx.x] = a2;
+#pragma unroll
+  for (int k = 0; k < K; ++k) s[k][threadIdx.x] = a[k];
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
-      s[0][threadIdx.x] += s[0][threadIdx.x + w];
-      s[1][threadIdx.x] += s[1][threadIdx.x + w];
-      s[2][threadIdx.x] += s[2][threadIdx.x + w];
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    double m0, m1;
     if (mode == 0) {
       c->sum_rho = s[0][0];
       c->mu[0] = s[1][0] / s[0][0];
       c->mu[1] = s[2][0] / s[0][0];
+      return;
+    } else if (mode == 1) {
+      m0 = s[0][0] / ncells;
+      m1 = s[1][0] / ncells;
     } else {
-      const double m0 = s[0][0] / ncells, m1 = s[1][0] / ncells;
-      const double dedt = c->power;
-      const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
-      double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
-      if (!(m0 > 1e-30)) scale = __builtin_nan("");
-      c->scale = scale;
+      const double mu0 = s[1][0] / s[0][0], mu1 = s[2][0] / s[0][0];
+      c->sum_rho = s[0][0];
+      c->mu[0] = mu0;
+      c->mu[1] = mu1;
+      const double var = s[3][0] - (mu0 * s[1][0] + mu1 * s[2][0]);
+      m0 = 0.5 * c->dt * var / ncells;
+      m1 = (s[4][0] - (mu0 * s[5][0] + mu1 * s[6][0])) / ncells;
     }
+    const double dedt = c->power;
+    const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
+    double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
+    if (!(m0 > 1e-30)) scale = __builtin_nan("");
+    c->scale = scale;
   }
 }
 

@@ -337,7 +337,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
   const T* src = first ? in : A;
   const size_t nblocks = (size_t)s.grid.x * s.grid.y;
   if (raw) {
-    int rc = ensure_partial(c, std::max<size_t>(nblocks, 4096) * 3);
+    int rc = ensure_partial(c, std::max<size_t>(nblocks, 4096) * jx::kick_sums<T, true>());
     if (rc) return rc;
   }
   for (int k = 0; k < ns; ++k) {
@@ -415,8 +415,9 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
   if (raw) {
     // Kick: forcing.py:137-149 then hydro.py:132, on the post-drag state in A.
     const size_t n = (size_t)p->nx * p->ny;
+    const bool one_pass = s.visc && jx::kick_sums<T, true>() == 7;   // as the stage kernel
     hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
-                       c->partial, (int)nblocks, 0, (double)n);
+                       c->partial, (int)nblocks, one_pass ? 2 : 0, (double)n);
     jx::KickArgs<T> ka{};
     ka.u = A;
     ka.raw = raw;
@@ -430,10 +431,12 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
     ka.ctrl = c->ctrl;
     ka.partial = c->partial;
     const dim3 g = aux_grid(n);
-    hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream,
-                       ka);
-    hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
-                       c->partial, (int)g.x, 1, (double)n);
+    if (!one_pass) {
+      hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0,
+                         c->stream, ka);
+      hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+                         c->partial, (int)g.x, 1, (double)n);
+    }
     ka.dtkey = want_dtkey ? &c->ctrl->dtkey : nullptr;
     hipLaunchKernelGGL(jx::kick_apply_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
   }

@@ -77,6 +77,40 @@ __device__ __forceinline__ void block_reduce_store(double v0, double v1, double 
   }
 }
 
+// The same for N values (out[0 .. N-1]); sred holds N doubles per wave.
+template <int N>
+__device__ __forceinline__ void block_reduce_store_n(double (&v)[N], double* sred, double* out) {
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += __shfl_xor(v[k], off);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) sred[N * wave + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      double a = 0;
+      for (int w = 0; w < nw; ++w) a += sred[N * w + k];
+      out[k] = a;
+    }
+  }
+}
+
+// Kick sums per block written by a final stage that feeds the kick.  FP64: rho, rho r
+// (the mean force; the second moments take another pass over the state, in the reference's
+// form).  FP32 viscous kernels (the driven-turbulence configuration): also rho |r|^2, m.r
+// and m, from which the second moments follow in one pass (forcing.py:141-145 expanded
+// about the mean, summed in float64).  The inviscid FP32 kernels keep three sums: four more
+// float64 accumulators would cost them a wave of occupancy.
+template <typename T, bool VISC> constexpr int kick_sums() {
+  return (Fast<T>::value && VISC) ? 7 : 3;
+}
+
 __device__ __forceinline__ void block_min_key(unsigned long long k, unsigned long long* sred,
                                               unsigned long long* slot) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -154,7 +188,10 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
     for (int k = 0; k < R; ++k) w[k][v] = u[k][v] = T(0);
   }
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  constexpr int KS = kick_sums<T, VISC>();
+  double acc[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) acc[k] = 0.0;
   CflMin<T> cfl(a.dx, a.dy, a.cs);
   T pf[NV];     // prefetched conserved values of the next row
   // Addresses are a wave-uniform 64-bit row base plus a 32-bit per-lane column offset.
@@ -422,9 +459,22 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         if (a.partial) {
           // The kick normalises against the post-drag state; the floor comes after it.
           const T* rf = a.raw + (size_t)j * nx * 2 + (x0 + t - 3) * 2;
-          acc0 += (double)un[0];
-          acc1 += (double)(un[0] * rf[0]);
-          acc2 += (double)(un[0] * rf[1]);
+          if constexpr (KS == 7) {
+            // float products are exact in float64
+            const double rho = un[0], r0 = rf[0], r1 = rf[1];
+            const double m0 = un[1], m1 = un[2];
+            acc[0] += rho;
+            acc[1] += rho * r0;
+            acc[2] += rho * r1;
+            acc[3] += rho * (r0 * r0 + r1 * r1);
+            acc[4] += m0 * r0 + m1 * r1;
+            acc[5] += m0;
+            acc[6] += m1;
+          } else {
+            acc[0] += (double)un[0];
+            acc[1] += (double)(un[0] * rf[0]);
+            acc[2] += (double)(un[0] * rf[1]);
+          }
         } else {
           un[0] = floor_to(un[0], dfloor);
           if (a.dtkey) {
@@ -453,8 +503,8 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   }
 
   if (a.partial) {
-    block_reduce_store<T>(acc0, acc1, acc2, sred,
-                          a.partial + 3 * (size_t)(blockIdx.y * gridDim.x + blockIdx.x));
+    block_reduce_store_n<KS>(acc, sred,
+                             a.partial + KS * (size_t)(blockIdx.y * gridDim.x + blockIdx.x));
   } else if (a.dtkey) {
     block_min_key(cfl.key(), reinterpret_cast<unsigned long long*>(sred), a.dtkey);
   }

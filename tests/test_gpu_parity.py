@@ -242,6 +242,35 @@ def test_fp32_render_by_mode_groups(jx, ref_vectors):
         assert np.max(np.abs(got - want)) <= 8 * M * 2.0 ** -23 * 2.0, shared
 
 
+@pytest.mark.parametrize("viscosity", (1.5e-3, 0.0))
+def test_fp32_driven_run_tracks_fp64(jx, viscosity):
+    """Driven runs in FP32 against FP64 from the same state and OU table.  Viscous FP32
+    kernels take the kick's second moments from one pass (sums about zero, float64);
+    inviscid ones from a second pass about the mean, as FP64 does.  Both must keep the kick
+    scale, so the runs agree to FP32 rounding."""
+    from jathenax_amd import forcing as jf
+    nx, ny = 48, 40
+    xs = (np.arange(nx) + 0.5) / nx - 0.5
+    ys = (np.arange(ny) + 0.5) / ny - 0.5
+    modes = jf.cartesian_modes(1, 3, (-3, 3), (-3, 3))
+    coef = jf.modal_coefficients(modes, (1.0, 1.0), 1, 3, 2.0, 1.0, 1618, 0.75, 0.004, 40)
+    U0 = np.zeros((ny, nx, 3))
+    U0[..., 0] = 1.0 + 0.1 * np.sin(2 * np.pi * xs)[None, :]
+    U0[..., 1] = 0.05 * U0[..., 0]    # a mean flow: the mean force and momentum are not zero
+    out = {}
+    for dt_ in (np.float64, np.float32):
+        basis = jf.forcing_basis(modes, xs.astype(dt_), ys.astype(dt_), (1.0, 1.0))
+        tdt = torch.float64 if dt_ == np.float64 else torch.float32
+        out[dt_] = jx.evolve_isothermal(
+            torch.tensor(U0, dtype=tdt, device="cuda"), 1 / nx, 1 / ny, 1.0, 1e-10, "rk2", 0.3,
+            0.05, 20, viscosity=viscosity, drag_rate=0.3333335,
+            forcing=(coef.astype(dt_), basis), dt_update=0.004, tdriv_start=0.0, dedt=0.5,
+            reconstruct="wenoz", rsolver="hlle")
+    U64, U32 = _np(out[np.float64][0]), _np(out[np.float32][0]).astype(np.float64)
+    assert out[np.float64][3] == out[np.float32][3]
+    assert np.max(np.abs(U32 - U64)) <= 2e-5 * np.max(np.abs(U64))
+
+
 def test_fp32_tracks_fp64(jx):
     from jathenax_amd import problems
     U0 = problems.smooth_wave(128, 128)
