@@ -108,9 +108,8 @@ def ensemble_leg(n_inst):
     work = sum(workloads.zone_stages(s, int(cycles[iid])) for s, iid in zip(specs, ids))
     lat = np.asarray([finish[k] for k in range(len(ids))])
     zones = sum(s["U"].shape[0] * s["U"].shape[1] for s in specs)
-    # Every 32nd instance against its own single-grid run. In FP32 the two kernels agree to
-    # round-off (their compiled contractions differ, and forced runs reduce the kick sums
-    # in another order); in FP64 unforced runs agree bit for bit (tested).
+    # Every 32nd instance against its own single-grid run: unforced runs agree bit for bit
+    # (tested); forced runs to round-off, since the kick sums are reduced in another order.
     from jathenax_amd import run
     worst = {"unforced": 0.0, "forced": 0.0}
     same_cycles = 0
