@@ -179,10 +179,15 @@ __device__ __forceinline__ void rk_row(int ns, int k, double& g0, double& g1, do
 template <typename T>
 __device__ __forceinline__ void riemann_rt(int rs, T rl, T rr, T ul, T ur, T tl, T tr,
                                            const Gas<T>& g, T& f0, T& f1, T& f2) {
+  // FP32 Roe and LLF fluxes come doubled, as in the fused stage kernel (flux_x2).
   switch (rs) {
-    case RS_LLF: riemann<RS_LLF, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
-    case RS_HLLE: riemann<RS_HLLE, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
-    default: riemann<RS_ROE, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
+    case RS_LLF:
+      riemann_x<RS_LLF, false, flux_x2<RS_LLF, T>()>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2);
+      break;
+    case RS_HLLE: riemann_x<RS_HLLE, false, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2); break;
+    default:
+      riemann_x<RS_ROE, false, flux_x2<RS_ROE, T>()>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2);
+      break;
   }
 }
 
@@ -265,6 +270,11 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
   const T dfloor = I.dfloor;
   const T drag_coef = sdt * I.drag;
   const bool visc_on = I.visc_on != 0;
+  // Doubled fluxes (FP32 Roe and LLF): viscous terms at h = 1, update coefficients halved.
+  const bool x2 = Fast<T>::value && I.rs != RS_HLLE;
+  const T vh = x2 ? T(1) : T(0.5);
+  const T cx = flux_coef<T>(beta * dt, I.inv_dx, x2);
+  const T cy = flux_coef<T>(beta * dt, I.inv_dy, x2);
   constexpr bool floors = RECON == RECON_PPMX || RECON == RECON_WENOZ;
 
   T w[R][NV], u[R][NV], hy[NV], gp[NV], gn[NV], pf[NV], ip[NV];
@@ -365,7 +375,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
           const T vx = over_rt((rs[128 + tp1] + rm[128 + tp1]) - (rs[128 + tm1] + rm[128 + tm1]),
                                I.four_dx, I.inv_four_dx);
           T fmx_, fmy_;
-          visc_face<0>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
+          visc_face_h<0>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, vh, fmx_, fmy_);
           gn[1] = gn[1] + fmx_;
           gn[2] = gn[2] + fmy_;
         }
@@ -459,7 +469,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
           const T vy = over_rt((rp[128 + t] + rp[128 + tm1]) - (rq[128 + t] + rq[128 + tm1]),
                                I.four_dy, I.inv_four_dy);
           T fmx_, fmy_;
-          visc_face<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
+          visc_face_h<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, vh, fmx_, fmy_);
           fx[1] = fx[1] + fmx_;
           fx[2] = fx[2] + fmy_;
         }
@@ -481,16 +491,27 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
             g_sout[cell * nv + v] = us[v];
           }
         }
+        if constexpr (Fast<T>::value) {
+          T dF[NV], dG[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const T div = over_rt(sF[v * 64 + tp1] - fx[v], I.dx, I.inv_dx) +
-                        over_rt(gn[v] - gp[v], I.dy, I.inv_dy);
-          if (blended)
-            un[v] = us[v] + (T(g0) * (u[S3][v] - us[v]) - sdt * div);
-          else
-            un[v] = (T(g0) * u[S3][v] + T(g1) * us[v]) - sdt * div;
+          for (int v = 0; v < NV; ++v) {
+            dF[v] = sF[v * 64 + tp1] - fx[v];
+            dG[v] = gn[v] - gp[v];
+          }
+          rk_update_fast<NV>(u[S3], us, dF, dG, k == 0, blended, T(g0), T(g1), cx, cy, un);
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            const T div = over_rt(sF[v * 64 + tp1] - fx[v], I.dx, I.inv_dx) +
+                          over_rt(gn[v] - gp[v], I.dy, I.inv_dy);
+            if (blended)
+              un[v] = us[v] + (T(g0) * (u[S3][v] - us[v]) - sdt * div);
+            else
+              un[v] = (T(g0) * u[S3][v] + T(g1) * us[v]) - sdt * div;
+          }
         }
         if (I.drag_on) {
+          __asm__ volatile("");   // a branch, not selects (as in the fused stage kernel)
           const T c = drag_coef * w[S3][0];
           un[1] = un[1] + (-(c * w[S3][1]));
           un[2] = un[2] + (-(c * w[S3][2]));
@@ -524,10 +545,9 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
         acc0 = acc1 = acc2 = 0.0;
       }
     }
-    if (i >= 5) {
+    // gn is set from iteration 5 on and gp first read at iteration 6.
 #pragma unroll
-      for (int v = 0; v < NV; ++v) gp[v] = gn[v];
-    }
+    for (int v = 0; v < NV; ++v) gp[v] = gn[v];
   };
 
   for (int i0 = 0; i0 < nit; i0 += R) {

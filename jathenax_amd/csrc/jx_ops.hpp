@@ -309,8 +309,10 @@ __device__ __forceinline__ T roe_average(T rl, T rr, T ul, T ur) {
 //    the Roe property A(Ur - Ul) = Fr - Fl make the upwind sum equal fl (or fr) up to
 //    rounding;
 //  * the LLF fallback (negative intermediate or right density) is rare, so one wave-wide
-//    vote skips it.
-template <typename T>
+//    vote skips it; the empty volatile asm keeps the compiler from turning the skipped
+//    block back into selects that every face would execute.
+// X2: return twice the flux (the caller folds the halving into its flux coefficient).
+template <bool X2, typename T>
 __device__ __forceinline__ void roe_fast(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
                                          T& f0, T& f1, T& f2) {
   const T cs = g.cs;
@@ -332,6 +334,7 @@ __device__ __forceinline__ void roe_fast(T rl, T rr, T ul, T ur, T tl, T tr, con
   T m2 = (s2 - AB * vt) - fabs(vn) * ash;
   const bool bad = (rl + am < T(0)) || (rr < T(0));
   if (__builtin_amdgcn_ballot_w64(bad) != 0) {
+    __asm__ volatile("");
     if (bad) {
       const T sp = fmx(fabs(ul) + cs, fabs(ur) + cs);
       m0 = s0 - sp * drho;
@@ -339,14 +342,30 @@ __device__ __forceinline__ void roe_fast(T rl, T rr, T ul, T ur, T tl, T tr, con
       m2 = s2 - sp * dmt;
     }
   }
-  f0 = T(0.5) * m0;
-  f1 = T(0.5) * m1;
-  f2 = T(0.5) * m2;
+  if constexpr (X2) {
+    f0 = m0;
+    f1 = m1;
+    f2 = m2;
+  } else {
+    f0 = T(0.5) * m0;
+    f1 = T(0.5) * m1;
+    f2 = T(0.5) * m2;
+  }
 }
 
-template <int RS, bool EXACT, typename T>
-__device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
-                                        T& f0, T& f1, T& f2) {
+// The FP32 fused and batched stage kernels take the Roe and LLF fluxes at twice their
+// value: the solvers' final halving is folded into the update's flux coefficient (and the
+// viscous and learned-stress face terms are doubled to match, both exactly).  FP64 and
+// HLLE return the flux itself.
+template <int RS, typename T> constexpr bool flux_x2() {
+  return Fast<T>::value && RS != RS_HLLE;
+}
+
+// Riemann flux at one face; X2 (only where flux_x2<RS, T>()): twice the flux.
+template <int RS, bool EXACT, bool X2, typename T>
+__device__ __forceinline__ void riemann_x(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
+                                          T& f0, T& f1, T& f2) {
+  static_assert(!X2 || flux_x2<RS, T>(), "doubled flux only for FP32 Roe and LLF");
   const T cs = g.cs, cs2 = g.cs2;
   if constexpr (RS == RS_LLF) {
     // fluxes.py:175-183
@@ -356,6 +375,12 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
     const T s1 = (mnl * ul + mnr * ur) + cs2 * (rl + rr);
     const T s2 = mnl * tl + mnr * tr;
     const T sp = fmx(fabs(ul) + cs, fabs(ur) + cs);
+    if constexpr (X2) {
+      f0 = s0 - sp * (rr - rl);
+      f1 = s1 - sp * (mnr - mnl);
+      f2 = s2 - sp * (mtr - mtl);
+      return;
+    }
     f0 = T(0.5) * (s0 - sp * (rr - rl));
     f1 = T(0.5) * (s1 - sp * (mnr - mnl));
     f2 = T(0.5) * (s2 - sp * (mtr - mtl));
@@ -395,7 +420,7 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
   } else {
     // Roe with supersonic upwinding and the LLF fallback, fluxes.py:134-172
     if constexpr (Fast<T>::value) {
-      roe_fast(rl, rr, ul, ur, tl, tr, g, f0, f1, f2);
+      roe_fast<X2>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2);
       return;
     }
     const T mnl = rl * ul, mnr = rr * ur;
@@ -444,6 +469,13 @@ __device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, cons
   }
 }
 
+// The flux itself (fluxes.py:204-219 per face).
+template <int RS, bool EXACT, typename T>
+__device__ __forceinline__ void riemann(T rl, T rr, T ul, T ur, T tl, T tr, const Gas<T>& g,
+                                        T& f0, T& f1, T& f2) {
+  riemann_x<RS, EXACT, false>(rl, rr, ul, ur, tl, tr, g, f0, f1, f2);
+}
+
 // ---------------------------------------------------------------------------------------
 // Viscous momentum flux at one face (fluxes.py:227-266).  Inputs are the face velocity
 // gradients dvx/dx, dvy/dy, dvx/dy, dvy/dx and rho_i + rho_(i-1).  Returns the x and y
@@ -463,9 +495,11 @@ __device__ __forceinline__ T smag_nu(T ux, T vy, T shear, T c) {
   return c * root(s2);
 }
 
+// h scales mu = (h nu) (rho_i + rho_(i-1)): 0.5 for the reference's flux, 1 for twice it
+// (exactly, as the scaling is by a power of two).
 template <int AXIS, typename T>
-__device__ __forceinline__ void visc_face(T ux, T vy, T uy, T vx, T rho_sum, const Visc<T>& p,
-                                          T& fmx_out, T& fmy_out) {
+__device__ __forceinline__ void visc_face_h(T ux, T vy, T uy, T vx, T rho_sum, const Visc<T>& p,
+                                            T h, T& fmx_out, T& fmy_out) {
   const T shear = vx + uy;    // dv_dx[..., 1] + dv_dy[..., 0]
   T sx, sy;
   if constexpr (AXIS == 1) {
@@ -477,9 +511,55 @@ __device__ __forceinline__ void visc_face(T ux, T vy, T uy, T vx, T rho_sum, con
   }
   T nu = p.nu;
   if (p.smag) nu = p.nu + smag_nu(ux, vy, shear, p.smag_c);
-  const T mu = (T(0.5) * nu) * rho_sum;
+  const T mu = (h * nu) * rho_sum;
   fmx_out = (-mu) * sx;
   fmy_out = (-mu) * sy;
+}
+
+// X2: twice the increments (h = 1; the compiler drops the multiply by one).
+template <int AXIS, bool X2 = false, typename T>
+__device__ __forceinline__ void visc_face(T ux, T vy, T uy, T vx, T rho_sum, const Visc<T>& p,
+                                          T& fmx_out, T& fmy_out) {
+  visc_face_h<AXIS>(ux, vy, uy, vx, rho_sum, p, X2 ? T(1) : T(0.5), fmx_out, fmy_out);
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 conservative update of one zone (hydro.py:39-51 with fluxes.py:283-287), shared by
+// the fused and the batched stage kernels so that both round alike:
+//   u' = (base - cx (F_(i+1) - F_i)) - cy (G_(j+1) - G_j),
+// with cx = beta dt / dx and cy = beta dt / dy (halved when the fluxes are doubled) and
+// base the RK row's combination of the stage input u and the saved register us: u itself
+// in stage 1 (every first row is (0, 1) and us == u there), us + g0 (u - us) for the
+// blended rows, g0 u + g1 us otherwise.  The uniform branches sit outside the per-variable
+// arithmetic.
+// ---------------------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T flux_coef(double beta_dt, T inv_d, bool x2) {
+  return T(beta_dt * (double)inv_d * (x2 ? 0.5 : 1.0));
+}
+
+template <int NV, typename T>
+__device__ __forceinline__ void rk_update_fast(const T (&u)[NV], const T (&us)[NV],
+                                               const T (&dF)[NV], const T (&dG)[NV],
+                                               bool stage1, bool blended, T g0, T g1, T cx,
+                                               T cy, T (&out)[NV]) {
+  T base[NV];
+  if (stage1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) base[v] = u[v];
+  } else if (blended) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) base[v] = us[v] + g0 * (u[v] - us[v]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) base[v] = g0 * u[v] + g1 * us[v];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const T t = base[v] - cx * dF[v];
+    out[v] = t - cy * dG[v];
+  }
 }
 
 // ---------------------------------------------------------------------------------------

@@ -175,6 +175,11 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   const Over<T, EXACT> over_4dx{a.four_dx, a.inv_four_dx}, over_4dy{a.four_dy, a.inv_four_dy};
   const T dfloor = a.dfloor;
   const T drag_coef = sdt * a.drag;
+  // FP32 Roe/LLF fluxes (and the viscous and stress terms added to them) are doubled; the
+  // update's flux coefficients take the halving.
+  constexpr bool X2 = flux_x2<RS, T>();
+  const T cx = flux_coef<T>(a.beta * dt, a.inv_dx, X2);
+  const T cy = flux_coef<T>(a.beta * dt, a.inv_dy, X2);
 
   T w[R][NV];   // primitives, row q in slot (q - y0 + 3) mod R
   T u[R][NV];   // floored conserved state, same slots
@@ -279,7 +284,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       if (i >= 5) {
         // y faces: normal = vy (index 2), tangent = vx (index 1) (fluxes.py:209-214)
         T f0, f1, f2;
-        riemann<RS, EXACT>(hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
+        riemann_x<RS, EXACT, X2>(hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
         gn[0] = f0;
         gn[1] = f2;
         gn[2] = f1;
@@ -295,7 +300,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           const T vx = over_4dx((rs[2 * BX + tp1] + rm[2 * BX + tp1]) -
                                 (rs[2 * BX + tm1] + rm[2 * BX + tm1]));
           T fmx_, fmy_;
-          visc_face<0>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
+          visc_face<0, X2>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
           gn[1] = gn[1] + fmx_;
           gn[2] = gn[2] + fmy_;
         }
@@ -307,8 +312,13 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           if (sm < 0) sm += ny;
           const T* t0 = a.tau + (size_t)s0 * nx * 3 + xc * 3;
           const T* tm = a.tau + (size_t)sm * nx * 3 + xc * 3;
-          gn[1] = gn[1] + T(0.5) * (tm[1] + t0[1]);
-          gn[2] = gn[2] + T(0.5) * (tm[2] + t0[2]);
+          if constexpr (X2) {
+            gn[1] = gn[1] + (tm[1] + t0[1]);
+            gn[2] = gn[2] + (tm[2] + t0[2]);
+          } else {
+            gn[1] = gn[1] + T(0.5) * (tm[1] + t0[1]);
+            gn[2] = gn[2] + T(0.5) * (tm[2] + t0[2]);
+          }
         }
       }
 #pragma unroll
@@ -389,7 +399,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       T fx[NV] = {};
       {
         T f0, f1, f2;
-        riemann<RS, EXACT>(l[0], xl[0], l[1], xl[1], l[2], xl[2], gas, f0, f1, f2);
+        riemann_x<RS, EXACT, X2>(l[0], xl[0], l[1], xl[1], l[2], xl[2], gas, f0, f1, f2);
         fx[0] = f0;
         fx[1] = f1;
         fx[2] = f2;
@@ -406,7 +416,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           const T vy = over_4dy((rp[2 * BX + t] + rp[2 * BX + tm1]) -
                                 (rq[2 * BX + t] + rq[2 * BX + tm1]));
           T fmx_, fmy_;
-          visc_face<1>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
+          visc_face<1, X2>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
           fx[1] = fx[1] + fmx_;
           fx[2] = fx[2] + fmy_;
         }
@@ -414,8 +424,13 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           // Learned stress on the low x face of column xc of row r-3: (tau_xx, tau_xy).
           const int xm = (xc == 0) ? nx - 1 : xc - 1;
           const T* trow = a.tau + (size_t)(r - 3) * nx * 3;
-          fx[1] = fx[1] + T(0.5) * (trow[xm * 3 + 0] + trow[xc * 3 + 0]);
-          fx[2] = fx[2] + T(0.5) * (trow[xm * 3 + 1] + trow[xc * 3 + 1]);
+          if constexpr (X2) {
+            fx[1] = fx[1] + (trow[xm * 3 + 0] + trow[xc * 3 + 0]);
+            fx[2] = fx[2] + (trow[xm * 3 + 1] + trow[xc * 3 + 1]);
+          } else {
+            fx[1] = fx[1] + T(0.5) * (trow[xm * 3 + 0] + trow[xc * 3 + 0]);
+            fx[2] = fx[2] + T(0.5) * (trow[xm * 3 + 1] + trow[xc * 3 + 1]);
+          }
         }
       }
 #pragma unroll
@@ -442,16 +457,29 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
             a.s_out[(size_t)j * row_len + lane_out + v] = us[v];
           }
         }
+        if constexpr (Fast<T>::value) {
+          T dF[NV], dG[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const T div = over_dx(sF[v * BX + tp1] - fx[v]) + over_dy(gn[v] - gp[v]);
-          if (a.blended)
-            un[v] = us[v] + (T(a.gam0) * (u[S3][v] - us[v]) - sdt * div);
-          else
-            un[v] = (T(a.gam0) * u[S3][v] + T(a.gam1) * us[v]) - sdt * div;
+          for (int v = 0; v < NV; ++v) {
+            dF[v] = sF[v * BX + tp1] - fx[v];
+            dG[v] = gn[v] - gp[v];
+          }
+          rk_update_fast<NV>(u[S3], us, dF, dG, a.stage1 != 0, a.blended != 0, T(a.gam0),
+                             T(a.gam1), cx, cy, un);
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            const T div = over_dx(sF[v * BX + tp1] - fx[v]) + over_dy(gn[v] - gp[v]);
+            if (a.blended)
+              un[v] = us[v] + (T(a.gam0) * (u[S3][v] - us[v]) - sdt * div);
+            else
+              un[v] = (T(a.gam0) * u[S3][v] + T(a.gam1) * us[v]) - sdt * div;
+          }
         }
         if (a.drag_on) {
-          // drag with the stage's incoming primitives (hydro.py:122-125)
+          // drag with the stage's incoming primitives (hydro.py:122-125); the empty
+          // volatile asm keeps this a branch rather than selects every zone would execute
+          __asm__ volatile("");
           const T c = drag_coef * w[S3][0];
           un[1] = un[1] + (-(c * w[S3][1]));
           un[2] = un[2] + (-(c * w[S3][2]));
@@ -486,10 +514,10 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         for (int v = 0; v < NV; ++v) out_row[lane_out + v] = un[v];
       }
     }
-    if (i >= 5) {
+    // gn is set from iteration 5 on and gp is first read at iteration 6, so the copy can
+    // be unconditional.
 #pragma unroll
-      for (int v = 0; v < NV; ++v) gp[v] = gn[v];
-    }
+    for (int v = 0; v < NV; ++v) gp[v] = gn[v];
   };
 
   for (int i0 = 0; i0 < nit; i0 += R) {
