@@ -29,6 +29,71 @@ template <typename T> __device__ __forceinline__ const gmem_t<T>* gmem(const T* 
   return (const gmem_t<T>*)p;
 }
 
+// Row access through a buffer resource.  The resource is built per row from the row's
+// wave-uniform base (scalar instructions only) and each lane adds a 32-bit byte offset, so
+// a row load or store costs no vector address arithmetic; a plain pointer access makes the
+// compiler fold the loop-invariant lane offset into a 64-bit vector base and add the row
+// offset to it with a 64-bit vector add per access.  num_records is the row's byte length
+// (an access past it reads zeros and writes nothing); dword 3 is the GFX9 raw-buffer
+// format word (32-bit data format).
+constexpr int kRawBufferWord3 = 0x00020000;
+
+template <typename T> __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const T* row,
+                                                                              int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), (short)0, bytes,
+                                           kRawBufferWord3);
+}
+
+// NV values of type T at byte offset off (NV * sizeof(T) bytes, 4-byte aligned), loaded
+// and stored in chunks of at most 16 bytes.
+template <typename T, int NV>
+__device__ __forceinline__ void row_load(__amdgpu_buffer_rsrc_t r, unsigned off, T (&out)[NV]) {
+  constexpr int W = NV * (int)sizeof(T) / 4;
+  unsigned d[W];
+  int k = 0;
+#pragma unroll
+  for (; k + 4 <= W; k += 4) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + 4 * k), 0, 0);
+    d[k] = x[0]; d[k + 1] = x[1]; d[k + 2] = x[2]; d[k + 3] = x[3];
+  }
+  if constexpr (W % 4 == 3) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(off + 4 * k), 0, 0);
+    d[k] = x[0]; d[k + 1] = x[1]; d[k + 2] = x[2];
+  } else if constexpr (W % 4 == 2) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + 4 * k), 0, 0);
+    d[k] = x[0]; d[k + 1] = x[1];
+  } else if constexpr (W % 4 == 1) {
+    d[k] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 4 * k), 0, 0);
+  }
+  __builtin_memcpy(out, d, sizeof(d));
+}
+
+template <typename T, int NV>
+__device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                          const T (&in)[NV]) {
+  constexpr int W = NV * (int)sizeof(T) / 4;
+  unsigned d[W];
+  __builtin_memcpy(d, in, sizeof(d));
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u3 __attribute__((ext_vector_type(3)));
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  int k = 0;
+#pragma unroll
+  for (; k + 4 <= W; k += 4) {
+    const u4 x = {d[k], d[k + 1], d[k + 2], d[k + 3]};
+    __builtin_amdgcn_raw_buffer_store_b128(x, r, (int)(off + 4 * k), 0, 0);
+  }
+  if constexpr (W % 4 == 3) {
+    const u3 x = {d[k], d[k + 1], d[k + 2]};
+    __builtin_amdgcn_raw_buffer_store_b96(x, r, (int)(off + 4 * k), 0, 0);
+  } else if constexpr (W % 4 == 2) {
+    const u2 x = {d[k], d[k + 1]};
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, (int)(off + 4 * k), 0, 0);
+  } else if constexpr (W % 4 == 1) {
+    __builtin_amdgcn_raw_buffer_store_b32(d[k], r, (int)(off + 4 * k), 0, 0);
+  }
+}
+
 // The instance table (written by the host between rounds) and the work lists (written by
 // the control kernel, an earlier launch) are read-only while a worker kernel runs, so they
 // are read through the constant address space: uniform addresses then become scalar loads

@@ -198,17 +198,25 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
   for (int k = 0; k < KS; ++k) acc[k] = 0.0;
   CflMin<T> cfl(a.dx, a.dy, a.cs);
-  T pf[NV];     // prefetched conserved values of the next row
-  // Addresses are a wave-uniform 64-bit row base plus a 32-bit per-lane column offset.
+  T pf[NV];     // prefetched conserved values of the next row (r+1)
+  T pf2[NV];    // and of the row after it (r+2)
+  T svn[NV];    // saved register of the row the next iteration updates
   const size_t row_len = (size_t)nx * NV;
-  const int lane_in = xc * NV;
-  const int lane_out = (x0 + t - 3) * NV;
+  const int row_bytes = (int)(row_len * sizeof(T));
+  // Rows are read and written through per-row buffer resources (row_load/row_store) at
+  // these 32-bit lane byte offsets.  lane_out wraps for the left halo lanes, which never
+  // store.
+  const unsigned lane_in = (unsigned)(xc * NV) * (unsigned)sizeof(T);
+  const unsigned lane_out = (unsigned)((x0 + t - 3) * NV) * (unsigned)sizeof(T);
   {
     int yn = (y0 - 3) % ny;
     if (yn < 0) yn += ny;
-    const T* src = a.u_in + (size_t)yn * row_len;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
+    row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pf);
+    yn = (y0 - 2) % ny;
+    if (yn < 0) yn += ny;
+    row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pf2);
+    // iteration 0 updates nothing: any cached row will do
+    row_load<T, NV>(row_rsrc(a.u_in + (size_t)y0 * row_len, row_bytes), lane_in, svn);
   }
 
   auto body = [&](auto pc, int i) {
@@ -216,32 +224,36 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
     constexpr int S4 = (P + R - 4) % R;   // row r-4 (5-row ring only)
     const int r = y0 - 3 + i;
-    // Both global loads of the row are unconditional, so the memory-wait pass sees
-    // straight-line code and waits for each load alone. First the saved register of the
-    // row updated this iteration (pointwise; during the warm-up, and in stage 1 where it
-    // is unused, the block's first input row, which stays cached), then the prefetch of
-    // row r+1 (past the end, the wrapped row). Loads return in order, so the update
-    // waits for the first while the prefetch stays in flight.
+    // Loads run ahead of their use so that HBM latency under full load (longer than one
+    // row iteration) stays hidden: the input row two iterations ahead and the saved
+    // register one iteration ahead.  Both loads are unconditional, so the memory-wait
+    // pass sees straight-line code and waits for each load alone.  The saved register
+    // (pointwise) is the row the next iteration updates; during the warm-up, and in
+    // stage 1 where it is unused, the block's first input row, which stays cached.
     T svr[NV];
-    {
-      const T* sv = a.stage1 ? a.u_in + (size_t)y0 * row_len
-                             : a.u_sv + (size_t)(i >= 6 ? r - 3 : y0) * row_len;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) svr[v] = sv[lane_in + v];
+    for (int v = 0; v < NV; ++v) svr[v] = svn[v];
+    {
+      // Row r-2 is updated by iteration i+1; past the block's last row (the final
+      // iteration, where row yend may be row ny), the cached first row instead.
+      const T* sv = a.stage1 ? a.u_in + (size_t)y0 * row_len
+                             : a.u_sv + (size_t)((i >= 5 && r - 2 < yend) ? r - 2 : y0) * row_len;
+      row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svn);
     }
     {
-      // Row r was fetched one iteration ago; issue row r+1 now so its latency overlaps
-      // this row's arithmetic.
+      // Row r was fetched two iterations ago and row r+1 one; issue row r+2 now (past the
+      // end, the wrapped row).
 #pragma unroll
-      for (int v = 0; v < NV; ++v) u[S0][v] = pf[v];
-      int yn = r + 1;   // periodic row; only the 3-row halos wrap
+      for (int v = 0; v < NV; ++v) {
+        u[S0][v] = pf[v];
+        pf[v] = pf2[v];
+      }
+      int yn = r + 2;   // periodic row; only the 3-row halos wrap
       if (yn < 0 || yn >= ny) {
         yn %= ny;
         if (yn < 0) yn += ny;
       }
-      const T* src = a.u_in + (size_t)yn * row_len;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) pf[v] = src[lane_in + v];
+      row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pf2);
     }
     {
       // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
@@ -452,10 +464,8 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         if (a.has_delta) {
           // RK4 saved-register accumulation (hydro.py:42-43)
 #pragma unroll
-          for (int v = 0; v < NV; ++v) {
-            us[v] = us[v] + T(a.delta) * u[S3][v];
-            a.s_out[(size_t)j * row_len + lane_out + v] = us[v];
-          }
+          for (int v = 0; v < NV; ++v) us[v] = us[v] + T(a.delta) * u[S3][v];
+          row_store<T, NV>(row_rsrc(a.s_out + (size_t)j * row_len, row_bytes), lane_out, us);
         }
         if constexpr (Fast<T>::value) {
           T dF[NV], dG[NV];
@@ -510,8 +520,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
             cfl.add(inv * un[1], inv * un[2]);
           }
         }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) out_row[lane_out + v] = un[v];
+        row_store<T, NV>(row_rsrc(out_row, row_bytes), lane_out, un);
       }
     }
     // gn is set from iteration 5 on and gp is first read at iteration 6, so the copy can
