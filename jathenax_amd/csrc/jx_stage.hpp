@@ -136,7 +136,9 @@ __host__ __device__ constexpr size_t stage_smem_bytes(int nv, bool visc, int bx)
 
 template <int P> using IC = std::integral_constant<int, P>;
 
-template <typename T, int RECON, int RS, int NV, bool VISC, bool EXACT>
+// ST1: the first stage of a step (its RK row is (0, 1) and the saved register is the stage
+// input itself), which reads no saved register.
+template <typename T, int RECON, int RS, int NV, bool VISC, bool EXACT, bool ST1>
 __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   // A local copy keeps the lambda below from taking the kernel argument's address.
   const StageArgs<T> a = args;
@@ -198,9 +200,14 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
   for (int k = 0; k < KS; ++k) acc[k] = 0.0;
   CflMin<T> cfl(a.dx, a.dy, a.cs);
-  T pf[NV];     // prefetched conserved values of the next row (r+1)
-  T pf2[NV];    // and of the row after it (r+2)
-  T svn[NV];    // saved register of the row the next iteration updates
+  // Loads run ahead of their use so that HBM latency under full load (longer than one row
+  // iteration) stays hidden.  With the 4-row ring (even, so the loop parity is known at
+  // compile time) the input row is fetched two iterations ahead and the saved register
+  // one iteration ahead, each into a two-slot register ring indexed by parity (no register
+  // moves); with the 5-row ring, one row ahead and in the same iteration.
+  constexpr bool RING2 = (R % 2) == 0;
+  T pfq[2][NV];   // prefetched input rows: slot (i & 1) holds row r (RING2), pfq[0] row r+1
+  T svq[2][NV];   // saved register rows: iteration i updates from slot ((i + 1) & 1)
   const size_t row_len = (size_t)nx * NV;
   const int row_bytes = (int)(row_len * sizeof(T));
   // Rows are read and written through per-row buffer resources (row_load/row_store) at
@@ -208,15 +215,19 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   // store.
   const unsigned lane_in = (unsigned)(xc * NV) * (unsigned)sizeof(T);
   const unsigned lane_out = (unsigned)((x0 + t - 3) * NV) * (unsigned)sizeof(T);
+  const T* const sv_base = a.u_sv;
   {
     int yn = (y0 - 3) % ny;
     if (yn < 0) yn += ny;
-    row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pf);
-    yn = (y0 - 2) % ny;
-    if (yn < 0) yn += ny;
-    row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pf2);
-    // iteration 0 updates nothing: any cached row will do
-    row_load<T, NV>(row_rsrc(a.u_in + (size_t)y0 * row_len, row_bytes), lane_in, svn);
+    row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pfq[0]);
+    if constexpr (RING2) {
+      yn = (y0 - 2) % ny;
+      if (yn < 0) yn += ny;
+      row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pfq[1]);
+      // iteration 0 updates nothing: any row will do (the block's first, which stays cached)
+      if constexpr (!ST1)
+        row_load<T, NV>(row_rsrc(sv_base + (size_t)y0 * row_len, row_bytes), lane_in, svq[1]);
+    }
   }
 
   auto body = [&](auto pc, int i) {
@@ -224,36 +235,38 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
     constexpr int S4 = (P + R - 4) % R;   // row r-4 (5-row ring only)
     const int r = y0 - 3 + i;
-    // Loads run ahead of their use so that HBM latency under full load (longer than one
-    // row iteration) stays hidden: the input row two iterations ahead and the saved
-    // register one iteration ahead.  Both loads are unconditional, so the memory-wait
-    // pass sees straight-line code and waits for each load alone.  The saved register
-    // (pointwise) is the row the next iteration updates; during the warm-up, and in
-    // stage 1 where it is unused, the block's first input row, which stays cached.
+    // Both loads of an iteration are unconditional, so the memory-wait pass sees
+    // straight-line code and waits for each load alone.  The saved register (pointwise)
+    // is read for a row being updated; during the warm-up and past the block's last row
+    // (the final iteration, where row yend may be row ny), the block's first row, which
+    // stays cached.
+    constexpr int PA = P & 1, PB = (P + 1) & 1;
     T svr[NV];
+    if constexpr (!ST1) {
+      if constexpr (RING2) {
+        // row r-3 arrived last iteration; row r-2 is the next iteration's
 #pragma unroll
-    for (int v = 0; v < NV; ++v) svr[v] = svn[v];
-    {
-      // Row r-2 is updated by iteration i+1; past the block's last row (the final
-      // iteration, where row yend may be row ny), the cached first row instead.
-      const T* sv = a.stage1 ? a.u_in + (size_t)y0 * row_len
-                             : a.u_sv + (size_t)((i >= 5 && r - 2 < yend) ? r - 2 : y0) * row_len;
-      row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svn);
+        for (int v = 0; v < NV; ++v) svr[v] = svq[PB][v];
+        const T* sv = sv_base + (size_t)((i >= 5 && r - 2 < yend) ? r - 2 : y0) * row_len;
+        row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svq[PA]);
+      } else {
+        const T* sv = sv_base + (size_t)(i >= 6 ? r - 3 : y0) * row_len;
+        row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svr);
+      }
     }
     {
-      // Row r was fetched two iterations ago and row r+1 one; issue row r+2 now (past the
-      // end, the wrapped row).
+      // RING2: row r was fetched two iterations ago into this slot; issue row r+2 into
+      // it.  Otherwise row r came one iteration ago; issue row r+1.  Past the end, the
+      // wrapped row (only the 3-row halos wrap).
+      constexpr int PQ = RING2 ? PA : 0;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        u[S0][v] = pf[v];
-        pf[v] = pf2[v];
-      }
-      int yn = r + 2;   // periodic row; only the 3-row halos wrap
+      for (int v = 0; v < NV; ++v) u[S0][v] = pfq[PQ][v];
+      int yn = r + (RING2 ? 2 : 1);
       if (yn < 0 || yn >= ny) {
         yn %= ny;
         if (yn < 0) yn += ny;
       }
-      row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pf2);
+      row_load<T, NV>(row_rsrc(a.u_in + (size_t)yn * row_len, row_bytes), lane_in, pfq[PQ]);
     }
     {
       // recover_isothermal (hydro.py:54-60): floor, then (1/rho) * momentum.
@@ -453,7 +466,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         T* const out_row = a.u_out + (size_t)j * row_len;
         T un[NV] = {};
         T us[NV] = {};
-        if (a.stage1) {
+        if constexpr (ST1) {
 #pragma unroll
           for (int v = 0; v < NV; ++v) us[v] = u[S3][v];
         } else {
@@ -474,7 +487,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
             dF[v] = sF[v * BX + tp1] - fx[v];
             dG[v] = gn[v] - gp[v];
           }
-          rk_update_fast<NV>(u[S3], us, dF, dG, a.stage1 != 0, a.blended != 0, T(a.gam0),
+          rk_update_fast<NV>(u[S3], us, dF, dG, ST1, a.blended != 0, T(a.gam0),
                              T(a.gam1), cx, cy, un);
         } else {
 #pragma unroll
@@ -554,8 +567,12 @@ hipError_t launch_stage_e(int rs, int nv, bool visc, const StageArgs<T>& a, dim3
                           hipStream_t st) {
 #define JX_LAUNCH(RS_, NV_, V_)                                                             \
   do {                                                                                     \
-    hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_, EXACT>), grid, dim3(bx),      \
-                       stage_smem_bytes<T>(NV_, V_, bx), st, a);                           \
+    if (a.stage1)                                                                          \
+      hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_, EXACT, true>), grid,        \
+                         dim3(bx), stage_smem_bytes<T>(NV_, V_, bx), st, a);               \
+    else                                                                                   \
+      hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_, EXACT, false>), grid,       \
+                         dim3(bx), stage_smem_bytes<T>(NV_, V_, bx), st, a);               \
     return hipGetLastError();                                                              \
   } while (0)
 #define JX_NV(RS_)                                                                          \
@@ -579,7 +596,7 @@ int occupancy_e(int rs, int nv, bool visc, int bx) {
 #define JX_OCC(RS_, NV_, V_)                                                                \
   do {                                                                                     \
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                       \
-            &nb, stage_kernel<T, RECON, RS_, NV_, V_, EXACT>, bx,                          \
+            &nb, stage_kernel<T, RECON, RS_, NV_, V_, EXACT, false>, bx,                   \
             stage_smem_bytes<T>(NV_, V_, bx)) != hipSuccess)                               \
       nb = 0;                                                                              \
     return nb;                                                                             \
