@@ -230,8 +230,12 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     }
   }
 
-  auto body = [&](auto pc, int i) {
+  // WARM: one of the first six iterations, where the row ring, the y faces and the x work
+  // start up in turn; in the steady iterations (i >= 6) every past(k) test folds to true.
+  auto body = [&](auto pc, auto warm, int i) {
     constexpr int P = decltype(pc)::value;
+    constexpr bool WARM = decltype(warm)::value;
+    auto past = [&](int k) { return !WARM || i >= k; };
     constexpr int S0 = P, S1 = (P + R - 1) % R, S2 = (P + R - 2) % R, S3 = (P + R - 3) % R;
     constexpr int S4 = (P + R - 4) % R;   // row r-4 (5-row ring only)
     const int r = y0 - 3 + i;
@@ -247,10 +251,10 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         // row r-3 arrived last iteration; row r-2 is the next iteration's
 #pragma unroll
         for (int v = 0; v < NV; ++v) svr[v] = svq[PB][v];
-        const T* sv = sv_base + (size_t)((i >= 5 && r - 2 < yend) ? r - 2 : y0) * row_len;
+        const T* sv = sv_base + (size_t)((past(5) && r - 2 < yend) ? r - 2 : y0) * row_len;
         row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svq[PA]);
       } else {
-        const T* sv = sv_base + (size_t)(i >= 6 ? r - 3 : y0) * row_len;
+        const T* sv = sv_base + (size_t)(past(6) ? r - 3 : y0) * row_len;
         row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svr);
       }
     }
@@ -278,18 +282,18 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       for (int v = 1; v < NV; ++v) w[S0][v] = inv * u[S0][v];
     }
     if constexpr (VISC) {
-      if (i >= 2) {
+      if (past(2)) {
         T* slot = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;
         slot[t] = w[S2][0];
         slot[BX + t] = w[S2][1];
         slot[2 * BX + t] = w[S2][2];
       }
     }
-    if (i >= 6) {
+    if (past(6)) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v * BX + t] = w[S3][v];
     }
-    if (VISC ? (i >= 2) : (i >= 6)) __syncthreads();   // B1
+    if (VISC ? (past(2)) : (past(6))) __syncthreads();   // B1
     // The x stencil of row r-3 is read now, so that the LDS latency overlaps the
     // y-direction arithmetic below.
     T xm2[NV], xm1[NV], xp1[NV], xp2[NV];
@@ -306,7 +310,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 
     // ---- y direction: edges of row r-2, flux at its low face -------------------------
     auto y_face = [&](T (&lo)[NV], T (&hi)[NV]) {
-      if (i >= 5) {
+      if (past(5)) {
         // y faces: normal = vy (index 2), tangent = vx (index 1) (fluxes.py:209-214)
         T f0, f1, f2;
         riemann_x<RS, EXACT, X2>(hy[0], lo[0], hy[2], lo[2], hy[1], lo[1], gas, f0, f1, f2);
@@ -350,12 +354,12 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
       for (int v = 0; v < NV; ++v) hy[v] = hi[v];
     };
     if constexpr (SHARED) {
-      if (i >= 3) {
+      if (past(3)) {
         T in[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
           in[v] = ppm4_interface(w[S3][v], w[S2][v], w[S1][v], w[S0][v]);   // below row r-1
-        if (i >= 4) {
+        if (past(4)) {
           T lo[NV], hi[NV];
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
@@ -369,7 +373,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         for (int v = 0; v < NV; ++v) ip[v] = in[v];
       }
     } else {
-      if (i >= 4) {
+      if (past(4)) {
         T lo[NV], hi[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
@@ -384,7 +388,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     }
 
     // ---- x direction for row j = r-3, then the update --------------------------------
-    if (i >= 6) {
+    if (past(6)) {
       T xl[NV] = {}, xh[NV] = {};
       T l[NV] = {};   // left state at the cell's low x face: the left neighbour's high edge
       if constexpr (SHARED) {
@@ -542,13 +546,22 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     for (int v = 0; v < NV; ++v) gp[v] = gn[v];
   };
 
-  for (int i0 = 0; i0 < nit; i0 += R) {
-    body(IC<0>{}, i0);
-    if (i0 + 1 < nit) body(IC<1>{}, i0 + 1);
-    if (i0 + 2 < nit) body(IC<2>{}, i0 + 2);
-    if (i0 + 3 < nit) body(IC<3>{}, i0 + 3);
+  // nit = rows + 6 >= 7: the six warm-up iterations always run.
+  using Warm = std::true_type;
+  using Steady = std::false_type;
+  body(IC<0>{}, Warm{}, 0);
+  body(IC<1>{}, Warm{}, 1);
+  body(IC<2>{}, Warm{}, 2);
+  body(IC<3>{}, Warm{}, 3);
+  body(IC<4 % R>{}, Warm{}, 4);
+  body(IC<5 % R>{}, Warm{}, 5);
+  for (int i0 = 6; i0 < nit; i0 += R) {
+    body(IC<6 % R>{}, Steady{}, i0);
+    if (i0 + 1 < nit) body(IC<7 % R>{}, Steady{}, i0 + 1);
+    if (i0 + 2 < nit) body(IC<8 % R>{}, Steady{}, i0 + 2);
+    if (i0 + 3 < nit) body(IC<9 % R>{}, Steady{}, i0 + 3);
     if constexpr (R == 5) {
-      if (i0 + 4 < nit) body(IC<4 % R>{}, i0 + 4);
+      if (i0 + 4 < nit) body(IC<10 % R>{}, Steady{}, i0 + 4);
     }
   }
 
