@@ -129,17 +129,30 @@ __device__ __forceinline__ void block_min_key(unsigned long long k, unsigned lon
   }
 }
 
+// LDS layout of the stage kernel.  Each region holds one row of the block's BX columns
+// plus two pad columns on either side, with a column's values interleaved at an odd word
+// stride (lds_stride): neighbouring lanes then fall in different banks, and every access
+// is one base register per region plus a compile-time offset (column shift times the
+// stride plus the variable).  Regions: primitives, high edges, face fluxes, PPM4
+// interfaces (lds_stride words per column each), then three rows of (rho, vx, vy) for the
+// viscous faces, then 64 doubles for the block reductions.
+__host__ __device__ constexpr int lds_stride(int nv) { return nv | 1; }
 template <typename T>
 __host__ __device__ constexpr size_t stage_smem_bytes(int nv, bool visc, int bx) {
-  return (size_t)(4 * nv + (visc ? 9 : 0)) * bx * sizeof(T) + 64 * sizeof(double);
+  return (size_t)(4 * lds_stride(nv) + (visc ? 9 : 0)) * (bx + 4) * sizeof(T) +
+         64 * sizeof(double);
 }
+// Pad columns on either side of each region.  They are set to 1 once per block: only
+// halo lanes, whose results are never used, read them, and 1 keeps their values finite.
+constexpr int kLdsPad = 2;
 
 template <int P> using IC = std::integral_constant<int, P>;
 
 // ST1: the first stage of a step (its RK row is (0, 1) and the saved register is the stage
 // input itself), which reads no saved register.
 template <typename T, int RECON, int RS, int NV, bool VISC, bool EXACT, bool ST1>
-__global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
+__global__ void __launch_bounds__(kMaxBlock)
+stage_kernel(StageArgs<T> args) {
   // A local copy keeps the lambda below from taking the kernel argument's address.
   const StageArgs<T> a = args;
   if (a.ctrl->done) return;
@@ -150,12 +163,30 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   extern __shared__ double smem_d[];
   const int BX = blockDim.x;
   const int t = threadIdx.x;
-  T* sW = reinterpret_cast<T*>(smem_d);   // [NV][BX] primitives of the row being updated
-  T* sH = sW + NV * BX;                   // [NV][BX] x high edges
-  T* sF = sH + NV * BX;                   // [NV][BX] x face fluxes
-  T* sV = sF + NV * BX;                   // [3 rows][rho, vx, vy][BX] (viscous only)
-  T* sI = sF + (NV + (VISC ? 9 : 0)) * BX;   // [NV][BX] PPM4 x interfaces
-  double* sred = reinterpret_cast<double*>(sI + NV * BX);
+  // This lane's column in each LDS region (see stage_smem_bytes); column shift d and
+  // variable v of a region are at [d * NS + v].
+  constexpr int NS = lds_stride(NV);
+  const int LW = (BX + 2 * kLdsPad) * NS;   // words per region
+  T* const lds = reinterpret_cast<T*>(smem_d);
+  T* const sW = lds + (t + kLdsPad) * NS;   // primitives of the row being updated
+  T* const sH = sW + LW;                    // x high edges
+  T* const sF = sH + LW;                    // x face fluxes
+  T* const sI = sF + LW;                    // PPM4 x interfaces
+  // viscous rows: slot k (row q with q mod 3 == k) at sV + k * VW, (rho, vx, vy) per column
+  const int VW = (BX + 2 * kLdsPad) * 3;
+  T* const sV = lds + 4 * LW + (t + kLdsPad) * 3;
+  double* sred = reinterpret_cast<double*>(lds + 4 * LW + (VISC ? 3 * VW : 0));
+  {
+    // Pad columns: two on either side of every region, written once (lanes 0-3).
+    if (t < 2 * kLdsPad) {
+      const int c = (t < kLdsPad) ? t : BX + t;
+      for (int k = 0; k < 4; ++k)
+        for (int v = 0; v < NS; ++v) lds[k * LW + c * NS + v] = T(1);
+      if constexpr (VISC)
+        for (int k = 0; k < 3; ++k)
+          for (int v = 0; v < 3; ++v) lds[4 * LW + k * VW + c * 3 + v] = T(1);
+    }
+  }
 
   const int nx = a.nx, ny = a.ny;
   const int OX = BX - 6;
@@ -163,8 +194,8 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
   int xc = (x0 + t - 3) % nx;
   if (xc < 0) xc += nx;
   const bool col_out = (t >= 3) && (t < BX - 3) && (x0 + t - 3 < nx);
-  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0);
-  const int tp1 = min(t + 1, BX - 1), tp2 = min(t + 2, BX - 1);
+  // column shifts in LDS words
+  constexpr int M2 = -2 * NS, M1 = -NS, P1 = NS, P2 = 2 * NS;
   const int y0 = blockIdx.y * a.rows;
   const int yend = min(y0 + a.rows, ny);
   const int nit = (yend - y0) + 6;
@@ -283,15 +314,15 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     }
     if constexpr (VISC) {
       if (past(2)) {
-        T* slot = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;
-        slot[t] = w[S2][0];
-        slot[BX + t] = w[S2][1];
-        slot[2 * BX + t] = w[S2][2];
+        T* slot = sV + ((r - 2 + 3 * ny + 3) % 3) * VW;
+        slot[0] = w[S2][0];
+        slot[1] = w[S2][1];
+        slot[2] = w[S2][2];
       }
     }
     if (past(6)) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sW[v * BX + t] = w[S3][v];
+      for (int v = 0; v < NV; ++v) sW[v] = w[S3][v];
     }
     if (VISC ? (past(2)) : (past(6))) __syncthreads();   // B1
     // The x stencil of row r-3 is read now, so that the LDS latency overlaps the
@@ -300,11 +331,10 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
     {   // unconditional: before the first barrier these are in-bounds, unused LDS words
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const T* row = sW + v * BX;
-        xm2[v] = row[tm2];
-        xm1[v] = row[tm1];
-        xp1[v] = row[tp1];
-        if constexpr (!SHARED) xp2[v] = row[tp2];
+        xm2[v] = sW[M2 + v];
+        xm1[v] = sW[M1 + v];
+        xp1[v] = sW[P1 + v];
+        if constexpr (!SHARED) xp2[v] = sW[P2 + v];
       }
     }
 
@@ -321,13 +351,12 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         for (int v = 3; v < NV; ++v) gn[v] = f0 * ((f0 >= T(0)) ? hy[v] : lo[v]);
         if (VISC && a.visc_terms) {
           // low y face of row s = r-2, column xc (fluxes.py:227-238 with axis=0)
-          const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row s
-          const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row s-1
+          const T* rs = sV + ((r - 2 + 3 * ny + 3) % 3) * VW;   // row s
+          const T* rm = sV + ((r - 3 + 3 * ny + 3) % 3) * VW;   // row s-1
           const T uy = over_dy(w[S2][1] - w[S3][1]);     // normal derivatives
           const T vy = over_dy(w[S2][2] - w[S3][2]);
-          const T ux = over_4dx((rs[BX + tp1] + rm[BX + tp1]) - (rs[BX + tm1] + rm[BX + tm1]));
-          const T vx = over_4dx((rs[2 * BX + tp1] + rm[2 * BX + tp1]) -
-                                (rs[2 * BX + tm1] + rm[2 * BX + tm1]));
+          const T ux = over_4dx((rs[3 + 1] + rm[3 + 1]) - (rs[-3 + 1] + rm[-3 + 1]));
+          const T vx = over_4dx((rs[3 + 2] + rm[3 + 2]) - (rs[-3 + 2] + rm[-3 + 2]));
           T fmx_, fmy_;
           visc_face<0, X2>(ux, vy, uy, vx, w[S2][0] + w[S3][0], vp, fmx_, fmy_);
           gn[1] = gn[1] + fmx_;
@@ -400,15 +429,15 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           xl[v] = ppm4_interface(xm2[v], xm1[v], w[S3][v], xp1[v]);
-          sI[v * BX + t] = xl[v];
+          sI[v] = xl[v];
         }
         __syncthreads();   // B1b
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          T nlo = sI[v * BX + tm1], nhi = xl[v];
+          T nlo = sI[M1 + v], nhi = xl[v];
           ppm4_limit(xm1[v], nlo, nhi);
           l[v] = nhi;
-          xh[v] = sI[v * BX + tp1];
+          xh[v] = sI[P1 + v];
           ppm4_limit(w[S3][v], xl[v], xh[v]);
         }
       } else {
@@ -420,10 +449,10 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           xh[0] = fmx(xh[0], dfloor);
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sH[v * BX + t] = xh[v];
+        for (int v = 0; v < NV; ++v) sH[v] = xh[v];
         __syncthreads();   // B2
 #pragma unroll
-        for (int v = 0; v < NV; ++v) l[v] = sH[v * BX + tm1];
+        for (int v = 0; v < NV; ++v) l[v] = sH[M1 + v];
       }
       T fx[NV] = {};
       {
@@ -436,16 +465,15 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         for (int v = 3; v < NV; ++v) fx[v] = f0 * ((f0 >= T(0)) ? l[v] : xl[v]);
         if (VISC && a.visc_terms) {
           // low x face of row j = r-3 at column xc (fluxes.py:227-238 with axis=1)
-          const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * 3 * BX;   // row j
-          const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * 3 * BX;   // row j+1
-          const T* rq = sV + ((r - 4 + 3 * ny + 3) % 3) * 3 * BX;   // row j-1
-          const T ux = over_dx(rj[BX + t] - rj[BX + tm1]);
-          const T vx = over_dx(rj[2 * BX + t] - rj[2 * BX + tm1]);
-          const T uy = over_4dy((rp[BX + t] + rp[BX + tm1]) - (rq[BX + t] + rq[BX + tm1]));
-          const T vy = over_4dy((rp[2 * BX + t] + rp[2 * BX + tm1]) -
-                                (rq[2 * BX + t] + rq[2 * BX + tm1]));
+          const T* rj = sV + ((r - 3 + 3 * ny + 3) % 3) * VW;   // row j
+          const T* rp = sV + ((r - 2 + 3 * ny + 3) % 3) * VW;   // row j+1
+          const T* rq = sV + ((r - 4 + 3 * ny + 3) % 3) * VW;   // row j-1
+          const T ux = over_dx(rj[1] - rj[-3 + 1]);
+          const T vx = over_dx(rj[2] - rj[-3 + 2]);
+          const T uy = over_4dy((rp[1] + rp[-3 + 1]) - (rq[1] + rq[-3 + 1]));
+          const T vy = over_4dy((rp[2] + rp[-3 + 2]) - (rq[2] + rq[-3 + 2]));
           T fmx_, fmy_;
-          visc_face<1, X2>(ux, vy, uy, vx, rj[t] + rj[tm1], vp, fmx_, fmy_);
+          visc_face<1, X2>(ux, vy, uy, vx, rj[0] + rj[-3], vp, fmx_, fmy_);
           fx[1] = fx[1] + fmx_;
           fx[2] = fx[2] + fmy_;
         }
@@ -463,7 +491,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         }
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sF[v * BX + t] = fx[v];
+      for (int v = 0; v < NV; ++v) sF[v] = fx[v];
       __syncthreads();   // B3
       if (col_out) {
         const int j = r - 3;
@@ -488,7 +516,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
           T dF[NV], dG[NV];
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
-            dF[v] = sF[v * BX + tp1] - fx[v];
+            dF[v] = sF[P1 + v] - fx[v];
             dG[v] = gn[v] - gp[v];
           }
           rk_update_fast<NV>(u[S3], us, dF, dG, ST1, a.blended != 0, T(a.gam0),
@@ -496,7 +524,7 @@ __global__ void __launch_bounds__(kMaxBlock) stage_kernel(StageArgs<T> args) {
         } else {
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
-            const T div = over_dx(sF[v * BX + tp1] - fx[v]) + over_dy(gn[v] - gp[v]);
+            const T div = over_dx(sF[P1 + v] - fx[v]) + over_dy(gn[v] - gp[v]);
             if (a.blended)
               un[v] = us[v] + (T(a.gam0) * (u[S3][v] - us[v]) - sdt * div);
             else
