@@ -271,10 +271,11 @@ stage_kernel(StageArgs<T> args) {
     constexpr int S4 = (P + R - 4) % R;   // row r-4 (5-row ring only)
     const int r = y0 - 3 + i;
     // Both loads of an iteration are unconditional, so the memory-wait pass sees
-    // straight-line code and waits for each load alone.  The saved register (pointwise)
-    // is read for a row being updated; during the warm-up and past the block's last row
-    // (the final iteration, where row yend may be row ny), the block's first row, which
-    // stays cached.
+    // straight-line code and waits for each load alone.  Where an iteration needs no new
+    // row, it re-reads one that is still cached: the saved register during the warm-up
+    // (the block's first row, read by the input stream at the same time) and in the final
+    // iteration (row r-3, loaded the iteration before; row r-2 = yend may be row ny);
+    // the input stream past the block's top halo (row r, loaded two iterations ago).
     constexpr int PA = P & 1, PB = (P + 1) & 1;
     T svr[NV];
     if constexpr (!ST1) {
@@ -282,7 +283,8 @@ stage_kernel(StageArgs<T> args) {
         // row r-3 arrived last iteration; row r-2 is the next iteration's
 #pragma unroll
         for (int v = 0; v < NV; ++v) svr[v] = svq[PB][v];
-        const T* sv = sv_base + (size_t)((past(5) && r - 2 < yend) ? r - 2 : y0) * row_len;
+        const int q = (past(5) && r - 2 < yend) ? r - 2 : (past(6) ? r - 3 : y0);
+        const T* sv = sv_base + (size_t)q * row_len;
         row_load<T, NV>(row_rsrc(sv, row_bytes), lane_in, svq[PA]);
       } else {
         const T* sv = sv_base + (size_t)(past(6) ? r - 3 : y0) * row_len;
@@ -296,7 +298,8 @@ stage_kernel(StageArgs<T> args) {
       constexpr int PQ = RING2 ? PA : 0;
 #pragma unroll
       for (int v = 0; v < NV; ++v) u[S0][v] = pfq[PQ][v];
-      int yn = r + (RING2 ? 2 : 1);
+      // rows up to yend + 2 are needed
+      int yn = (r + (RING2 ? 2 : 1) <= yend + 2) ? r + (RING2 ? 2 : 1) : r;
       if (yn < 0 || yn >= ny) {
         yn %= ny;
         if (yn < 0) yn += ny;

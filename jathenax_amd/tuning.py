@@ -73,12 +73,14 @@ def batch_rows(nx, ny, default=32):
 def _single_candidates(nx, ny):
     # Block counts near a whole number of resident waves matter more than the shape
     # itself (a small second wave of blocks costs a whole block time), hence the dense
-    # band of heights around one full wave of 4096-wide grids.
+    # bands of heights around one full wave of 4096-wide grids at 4 and 5 resident
+    # blocks of 256 threads per CU.
     out = [(0, 0)]
     for bx in (256, 128):
         if bx > 64 * ((nx + 6 + 63) // 64) and bx != 128:
             continue
-        for rows in (8, 10, 12, 16, 24, 32, 35, 48, 64, 68, 69, 70, 80, 96):
+        for rows in (8, 10, 12, 16, 24, 28, 32, 35, 48, 55, 56, 58, 60, 64, 68, 69, 70, 80,
+                     96):
             if rows <= ny:
                 out.append((bx, rows))
     return out
