@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=2048, help="CPU baseline sample grid size")
     ap.add_argument("--block-x", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="also time plain streaming kernels over arrays of the state's size "
+                         "(read-two-write-one add, copy) for the practical HBM rate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 512 MiB device-to-device copy (PMC byte calibration)")
     ap.add_argument("--tune", action="store_true",
@@ -228,6 +231,31 @@ def cpu_leg(problems, n):
     return n * n * 3 / sec, sec
 
 
+def stream_leg(nelem, reps=50):
+    """HBM rate of plain torch streaming kernels over float32 arrays of nelem values: the
+    read-two-write-one pattern of RK stages 2-3 (c = a + b) and a copy."""
+    import torch
+    a = torch.rand(nelem, device="cuda")
+    b = torch.rand(nelem, device="cuda")
+    c = torch.empty_like(a)
+    res = {}
+    for name, fn, arrays in (("add_2r1w", lambda: torch.add(a, b, out=c), 3),
+                             ("copy_1r1w", lambda: c.copy_(a), 2)):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / reps * 1e-3
+        res[name] = {"GB_per_s": arrays * 4 * nelem / sec / 1e9, "us": sec * 1e6}
+    return res
+
+
 def read_traffic(n, dtype, tiles):
     """HBM bytes per stage launch for this grid, precision and tile shape, from the
     committed rocprofv3 --pmc summary; None when that configuration was not measured."""
@@ -315,6 +343,9 @@ def main():
     if a.serve > 0:
         print(f"# bench: serving leg ({a.serve} requests)", file=sys.stderr, flush=True)
         out["serving"] = serving_leg(a.serve, a.serve_rate)
+    if a.stream_ceiling:
+        print("# bench: streaming ceiling", file=sys.stderr, flush=True)
+        out["stream_ceiling"] = stream_leg(n * n * nvar)
     if not a.no_cpu_baseline:
         print("# bench: cpu baseline", file=sys.stderr, flush=True)
         rate, sec = cpu_leg(problems, a.cpu_n)
