@@ -113,6 +113,54 @@ def test_fp32_tile_shapes_are_bitwise_equal(jx, recon, solver):
         ctx.set_tiles(0, 0)
 
 
+@pytest.mark.parametrize("recon,solver", [("ppm4", "roe"), ("plm", "llf"), ("wenoz", "hlle")])
+def test_fp32_inviscid_runs_tile_shapes_are_bitwise_equal(jx, recon, solver):
+    """The inviscid kernels (the 4096^2 benchmark's path) over several steps: the first-stage
+    instantiation, the later stages and the CFL epilogue give the same bits for every tile,
+    including bands whose last one ends at row ny (110 = 2 x 55) and a ragged one (28)."""
+    from jathenax_amd.device import context_for
+    rng = np.random.default_rng(8)
+    ny, nx = 110, 131
+    U = np.empty((ny, nx, 3))
+    U[..., 0] = 1.0 + 0.3 * rng.random((ny, nx))
+    U[..., 1:] = 0.3 * rng.standard_normal((ny, nx, 2))
+    Ud = torch.tensor(U, dtype=torch.float32, device="cuda")
+    ctx = context_for(Ud)
+    args = (1 / nx, 1 / ny, 1.0, 1e-6, "rk3", 0.4, 10.0, 4)
+    kw = dict(reconstruct=recon, rsolver=solver)
+    ctx.set_tiles(0, 0)
+    base = jx.evolve_isothermal(Ud, *args, **kw)
+    try:
+        for bx, rows in ((64, 1), (64, 7), (128, 13), (192, 5), (256, 28), (256, 55), (256, 64)):
+            ctx.set_tiles(bx, rows)
+            got = jx.evolve_isothermal(Ud, *args, **kw)
+            np.testing.assert_array_equal(_np(got[0]), _np(base[0]), err_msg=str((bx, rows)))
+            assert got[1:] == base[1:], (bx, rows)
+    finally:
+        ctx.set_tiles(0, 0)
+
+
+@pytest.mark.parametrize("bx,rows", [(256, 55), (256, 28), (128, 55)])
+def test_fp64_inviscid_step_at_benchmark_tiles(jx, bx, rows):
+    """FP64 PPM4/Roe RK3 step against the oracle at the benchmark's tile shapes; the top band
+    ends at row ny (110 = 2 x 55), where the saved register is read one iteration ahead."""
+    from jathenax_amd.device import context_for
+    rng = np.random.default_rng(9)
+    ny, nx = 110, 300
+    U = np.empty((ny, nx, 3))
+    U[..., 0] = 1.0 + 0.3 * rng.random((ny, nx))
+    U[..., 1:] = 0.4 * rng.standard_normal((ny, nx, 2))
+    want = ox.isothermal_step(U, 1e-3, 1 / nx, 1 / ny, 1.0, 1e-10, "rk3")
+    Ud = torch.tensor(U, device="cuda")
+    ctx = context_for(Ud)
+    ctx.set_tiles(bx, rows)
+    try:
+        got = jx.isothermal_step(Ud, 1e-3, 1 / nx, 1 / ny, 1.0, 1e-10, "rk3")
+    finally:
+        ctx.set_tiles(0, 0)
+    assert _rel_inf(_np(got), want) <= 1e-13
+
+
 @pytest.mark.parametrize("tlim,nlim,tag", [(0.2, 100000, "tlim"), (10.0, 100, "n100")])
 def test_smooth_run_100_cycles(jx, ref_vectors, tlim, nlim, tag):
     U0 = ref_vectors["run_smooth_U0"]
