@@ -190,13 +190,26 @@ stage_kernel(StageArgs<T> args) {
 
   const int nx = a.nx, ny = a.ny;
   const int OX = BX - 6;
-  const int x0 = blockIdx.x * OX;
+  // XCD-aware tile order.  Workgroups go to the 8 XCDs round-robin by linear id, and each
+  // XCD has its own L2.  The ids one XCD receives take consecutive tiles in strip-fastest
+  // order, so horizontally adjacent strips of a band run on one XCD at the same time and
+  // the cache lines they share (their 6-column overlap) are fetched into that L2 once.
+  int strip, band;
+  {
+    const int nblk = gridDim.x * gridDim.y;
+    const int lin = blockIdx.x + gridDim.x * blockIdx.y;
+    const int xcd = lin & 7, m = lin >> 3;
+    const int g = xcd * (nblk >> 3) + min(xcd, nblk & 7) + m;   // a bijection of [0, nblk)
+    strip = g % gridDim.x;
+    band = g / gridDim.x;
+  }
+  const int x0 = strip * OX;
   int xc = (x0 + t - 3) % nx;
   if (xc < 0) xc += nx;
   const bool col_out = (t >= 3) && (t < BX - 3) && (x0 + t - 3 < nx);
   // column shifts in LDS words
   constexpr int M2 = -2 * NS, M1 = -NS, P1 = NS, P2 = 2 * NS;
-  const int y0 = blockIdx.y * a.rows;
+  const int y0 = band * a.rows;
   const int yend = min(y0 + a.rows, ny);
   const int nit = (yend - y0) + 6;
 
@@ -598,7 +611,7 @@ stage_kernel(StageArgs<T> args) {
 
   if (a.partial) {
     block_reduce_store_n<KS>(acc, sred,
-                             a.partial + KS * (size_t)(blockIdx.y * gridDim.x + blockIdx.x));
+                             a.partial + KS * (size_t)(band * gridDim.x + strip));
   } else if (a.dtkey) {
     block_min_key(cfl.key(), reinterpret_cast<unsigned long long*>(sred), a.dtkey);
   }
