@@ -191,17 +191,24 @@ stage_kernel(StageArgs<T> args) {
   const int nx = a.nx, ny = a.ny;
   const int OX = BX - 6;
   // XCD-aware tile order.  Workgroups go to the 8 XCDs round-robin by linear id, and each
-  // XCD has its own L2.  The ids one XCD receives take consecutive tiles in strip-fastest
-  // order, so horizontally adjacent strips of a band run on one XCD at the same time and
-  // the cache lines they share (their 6-column overlap) are fetched into that L2 once.
+  // XCD has its own L2.  Within each window of 8 bands (8 x gridDim.x consecutive ids), the
+  // ids XCD x receives take every strip of band 8k + x, so horizontally adjacent strips of
+  // a band run on one XCD at the same time and the cache lines they share (their 6-column
+  // overlap) are fetched into that L2 once.  Windows keep the band order, so blocks beyond
+  // the resident ones still take the last bands; a final partial window keeps the plain
+  // order.
   int strip, band;
   {
-    const int nblk = gridDim.x * gridDim.y;
-    const int lin = blockIdx.x + gridDim.x * blockIdx.y;
-    const int xcd = lin & 7, m = lin >> 3;
-    const int g = xcd * (nblk >> 3) + min(xcd, nblk & 7) + m;   // a bijection of [0, nblk)
-    strip = g % gridDim.x;
-    band = g / gridDim.x;
+    const int W = gridDim.x;
+    const int lin = blockIdx.x + W * blockIdx.y;
+    const int win = lin / (8 * W), within = lin - win * 8 * W;
+    if (8 * (win + 1) <= (int)gridDim.y) {
+      band = 8 * win + (within & 7);
+      strip = within >> 3;
+    } else {
+      band = 8 * win + within / W;
+      strip = within - (within / W) * W;
+    }
   }
   const int x0 = strip * OX;
   int xc = (x0 + t - 3) % nx;
