@@ -46,12 +46,20 @@ typedef struct jx_run_params {
   int ny, nx, nvar;          /* state shape; nvar in {3, 4, 5} */
   int dtype;                 /* 0 float64, 1 float32 */
   int reconstruct, rsolver;  /* JX_RECON_*, JX_RS_* */
-  int integrator;            /* RK_STAGES key: 2 = rk2, 3 = rk3, 4 = rk4 */
+  int integrator;            /* number of RK stages: with rk_rows NULL, 2 = rk2, 3 = rk3,
+                                4 = rk4 of the built-in table; else 1..JX_MAX_RK_STAGES */
   int nlim;                  /* cycle limit (evolve only) */
   double dx, dy, cs, dfloor, cfl, tlim;
   double viscosity, smagorinsky_cs, drag_rate;
   double dt_update, tdriv_start, dedt;   /* forcing schedule (evolve only) */
+  /* The integrator's rows of RK_STAGES (jathenax/hydro.py:20-36), read on every call as
+   * hydro.py:100 reads the table: `integrator` rows of (gam0, gam1, beta, delta), delta NaN
+   * where the reference row holds None.  NULL selects the built-in rk2/rk3/rk4 rows.  The
+   * batched engine accepts only tables equal to the built-in rows. */
+  const double* rk_rows;
 } jx_run_params;
+
+#define JX_MAX_RK_STAGES 8
 
 /* Forcing tables (jathenax/run.py:148-155): the OU coefficient table and the phase
  * factors of forcing_basis, all device arrays in the state dtype. */
@@ -128,7 +136,9 @@ int jx_normalize_force(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const v
  * learned-stress path (hydro.py:111-119) run the stage as separate launches that keep
  * every intermediate array; each entry point restates one reference array function with
  * the fused kernel's scalar operators.  Arrays are (ny, nx, nvar) unless stated. */
-/* reconstruct_faces (fluxes.py:124-131): left/right states at each cell's low face. */
+/* reconstruct_faces (fluxes.py:124-131): left/right states at each cell's low face.
+ * dfloor NaN skips the PPMX/WENO-Z face-density floor, which gives the registry functions
+ * dc/plm/ppm4/ppmx/wenoz_face_states (fluxes.py:6-117). */
 int jx_reconstruct_faces(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* W,
                          int axis, int reconstruct, double dfloor, void* left, void* right);
 /* isothermal_flux (fluxes.py:207-219). */
@@ -138,6 +148,14 @@ int jx_isothermal_flux(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const v
 int jx_viscous_momentum_flux(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* W,
                              double dx, double dy, double viscosity, double smagorinsky_cs,
                              int axis, void* out);
+/* face_velocity_gradients (fluxes.py:227-238): dv_dx and dv_dy, each (ny, nx, 2), at each
+ * cell's low face along axis (1 = x faces, 0 = y faces). */
+int jx_face_velocity_gradients(jx_ctx* ctx, int dtype, int ny, int nx, int nvar, const void* W,
+                               double dx, double dy, int axis, void* dv_dx, void* dv_dy);
+/* smagorinsky_viscosity (fluxes.py:241-245) of n faces: nu (n) from dv_dx, dv_dy (n, 2). */
+int jx_smagorinsky_viscosity(jx_ctx* ctx, int dtype, long long n, const void* dv_dx,
+                             const void* dv_dy, double dx, double dy, double smagorinsky_cs,
+                             void* nu);
 /* flux[..., 1:3] += add, add is (ncells, 2) (hydro.py:106-110, 117-119). */
 int jx_add_momentum_flux(jx_ctx* ctx, int dtype, long long ncells, int nvar, void* flux,
                          const void* add);

@@ -20,6 +20,7 @@ class RunParams(ctypes.Structure):
         ("cfl", c_double), ("tlim", c_double),
         ("viscosity", c_double), ("smagorinsky_cs", c_double), ("drag_rate", c_double),
         ("dt_update", c_double), ("tdriv_start", c_double), ("dedt", c_double),
+        ("rk_rows", POINTER(c_double)),
     ]
 
 
@@ -82,6 +83,10 @@ SIGNATURES = {
     "jx_viscous_momentum_flux": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
                                          c_double, c_double, c_double, c_double, c_int,
                                          c_void_p]),
+    "jx_face_velocity_gradients": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                           c_double, c_double, c_int, c_void_p, c_void_p]),
+    "jx_smagorinsky_viscosity": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
+                                         c_double, c_double, c_double, c_void_p]),
     "jx_add_momentum_flux": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p]),
     "jx_apply_drag": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p, c_double,
                               c_double, c_double]),

@@ -122,6 +122,9 @@ template <typename T> class Batch : public BatchBase {
     if (p->reconstruct < 0 || p->reconstruct > 4 || p->rsolver < 0 || p->rsolver > 2 ||
         p->integrator < 2 || p->integrator > 4)
       return fail("add: unknown operator id", JX_EKEY);
+    if (p->rk_rows && !rk_rows_are_builtin(p->rk_rows, p->integrator))
+      return fail("add: the batched engine runs the built-in rk2/rk3/rk4 rows; this RK table "
+                  "differs from them");
     for (double v : {p->viscosity, p->smagorinsky_cs, p->drag_rate, p->dedt})
       if (!std::isfinite(v) || v < 0.0) return fail("add: coefficients must be finite and >= 0");
     if (f && (!f->coefficients || !f->cos_x || !f->sin_x || !f->cos_y || !f->sin_y ||

@@ -157,21 +157,45 @@ dim3 aux_grid(size_t n) {
   return dim3((unsigned)b);
 }
 
-// RK rows (jathenax/hydro.py:20-36).
-struct RkRow { double g0, g1, beta, delta; int has_delta; };
-const RkRow kRk2[2] = {{0.0, 1.0, 1.0, 0.0, 0}, {0.5, 0.5, 0.5, 0.0, 0}};
-const RkRow kRk3[3] = {{0.0, 1.0, 1.0, 0.0, 0},
-                       {0.25, 0.75, 0.25, 0.0, 0},
-                       {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0.0, 0}};
-const RkRow kRk4[4] = {{0.0, 1.0, 1.193743905974738, 0.0, 0},
-                       {0.121098479554482, 0.721781678111411, 0.099279895495783,
-                        0.217683334308543, 1},
-                       {-3.843833699660025, 2.121209265338722, 1.131678018054042,
-                        1.065841341361089, 1},
-                       {0.546370891121863, 0.198653035682705, 0.310665766509336, 0.0, 1}};
+using jx::RkRow;
 
-const RkRow* rk_rows(int integrator) {
-  return integrator == 2 ? kRk2 : integrator == 3 ? kRk3 : integrator == 4 ? kRk4 : nullptr;
+const RkRow* builtin_rk_rows(int integrator) {
+  return (integrator >= 2 && integrator <= 4) ? jx::kBuiltinRk[integrator - 2] : nullptr;
+}
+
+// The rows a run applies: the caller's table (rk_rows, delta NaN = None) when given,
+// else the built-in table of `integrator` stages.  Empty on a bad table.
+struct RkTable {
+  RkRow row[JX_MAX_RK_STAGES];
+  int n = 0;
+};
+
+RkTable rk_table(const jx_run_params* p) {
+  RkTable t;
+  if (!p->rk_rows) {
+    const RkRow* b = builtin_rk_rows(p->integrator);
+    if (!b) return t;
+    for (int k = 0; k < p->integrator; ++k) t.row[k] = b[k];
+    t.n = p->integrator;
+    return t;
+  }
+  if (p->integrator < 1 || p->integrator > JX_MAX_RK_STAGES) return t;
+  for (int k = 0; k < p->integrator; ++k) {
+    const double* r = p->rk_rows + 4 * k;
+    if (!std::isfinite(r[0]) || !std::isfinite(r[1]) || !std::isfinite(r[2]) ||
+        std::isinf(r[3]))
+      return RkTable{};
+    const bool has_delta = !std::isnan(r[3]);
+    t.row[k] = RkRow{r[0], r[1], r[2], has_delta ? r[3] : 0.0, has_delta ? 1 : 0};
+  }
+  t.n = p->integrator;
+  return t;
+}
+
+// The first row reads no saved register when it is (0, 1, beta, None): the fused
+// first-stage instantiation (ST1) then applies.
+bool plain_first_row(const RkRow& r) {
+  return r.g0 == 0.0 && r.g1 == 1.0 && !r.has_delta;
 }
 
 int check_params(jx_ctx* c, const jx_run_params* p) {
@@ -183,7 +207,10 @@ int check_params(jx_ctx* c, const jx_run_params* p) {
   if (p->reconstruct < 0 || p->reconstruct > 4)
     return fail(c, JX_EKEY, "unknown reconstruction id");
   if (p->rsolver < 0 || p->rsolver > 2) return fail(c, JX_EKEY, "unknown Riemann solver id");
-  if (!rk_rows(p->integrator)) return fail(c, JX_EKEY, "unknown integrator");
+  if (rk_table(p).n == 0)
+    return p->rk_rows ? fail(c, JX_EINVAL, "RK table rows must be finite (delta NaN for None), "
+                                           "1 to 8 stages")
+                      : fail(c, JX_EKEY, "unknown integrator");
   const double nonneg[3] = {p->viscosity, p->smagorinsky_cs, p->drag_rate};
   const char* names[3] = {"viscosity", "smagorinsky_cs", "drag_rate"};
   for (int k = 0; k < 3; ++k)
@@ -329,8 +356,8 @@ template <typename T>
 int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
                 const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr) {
   const jx_run_params* p = s.p;
-  const RkRow* rows = rk_rows(p->integrator);
-  const int ns = p->integrator;
+  const RkTable tab = rk_table(p);
+  const int ns = tab.n;
   T* B = static_cast<T*>(c->buf[0]);
   T* C = static_cast<T*>(c->buf[1]);
   T* S = static_cast<T*>(c->buf[2]);
@@ -340,42 +367,34 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
     int rc = ensure_partial(c, std::max<size_t>(nblocks, 4096) * jx::kick_sums<T, true>());
     if (rc) return rc;
   }
+  // Register plan.  Stage outputs alternate B, C, B, ... and the last goes to A; a stage
+  // never reads its stencil from the buffer it writes.  The saved register is the step's
+  // input (floored on read when that is the caller's array) until a row with a delta
+  // accumulates it into S (hydro.py:42-43); later stages read S.  Saved-register reads and
+  // writes are pointwise, so they may share a buffer with the stage output.
+  //   rk2: src->B, B|src->A          rk3: src->B, B|src->C, C|src->A
+  //   rk4: src->B, B|src->C (S=src+d2 B), C|S->B (S+=d3 C), B|S->A (S+=0 B)
+  // A one-stage table whose input is A itself writes B and copies it to A.
+  bool saved_in_s = false;
+  const T* prev = src;
   for (int k = 0; k < ns; ++k) {
     StageArgs<T> a = s.base;
-    const RkRow& r = rows[k];
+    const RkRow& r = tab.row[k];
     a.gam0 = r.g0;
     a.gam1 = r.g1;
     a.beta = r.beta;
     a.delta = r.delta;
     a.has_delta = r.has_delta;
     a.blended = (r.g0 != 0.0 && r.g0 + r.g1 == 1.0) ? 1 : 0;
-    a.stage1 = (k == 0);
-    a.floor_saved = 0;
-    a.u_sv = nullptr;
-    a.s_out = nullptr;
-    // Register plan (every read of u_sv / write of s_out is pointwise):
-    //   rk2: src->B, B|src->A          rk3: src->B, B|src->C, C|src->A
-    //   rk4: src->B, B|src->C (S=src+d2 B), C|S->B (S+=d3 C), B|S->A (S+=0 B)
-    if (k == 0) {
-      a.u_in = src;
-      a.u_out = B;
-    } else if (ns == 2) {
-      a.u_in = B;
-      a.u_out = A;
-    } else if (ns == 3) {
-      a.u_in = (k == 1) ? B : C;
-      a.u_out = (k == 1) ? C : A;
-    } else {
-      a.u_in = (k == 1) ? B : (k == 2) ? C : B;
-      a.u_out = (k == 1) ? C : (k == 2) ? B : A;
-      a.s_out = S;
-    }
-    if (k > 0) {
-      const bool from_src = (ns != 4) || (k == 1);
-      a.u_sv = from_src ? src : S;
-      a.floor_saved = (from_src && first) ? 1 : 0;
-    }
     const bool last = (k == ns - 1);
+    a.stage1 = (k == 0 && plain_first_row(r)) ? 1 : 0;
+    a.u_in = prev;
+    a.u_out = (last && prev != A) ? A : ((k & 1) ? C : B);
+    a.u_sv = a.stage1 ? nullptr : (saved_in_s ? S : src);
+    a.floor_saved = (!a.stage1 && !saved_in_s && first) ? 1 : 0;
+    a.s_out = r.has_delta ? S : nullptr;
+    if (r.has_delta) saved_in_s = true;
+    prev = a.u_out;
     a.raw = nullptr;
     a.partial = nullptr;
     a.dtkey = nullptr;
@@ -411,6 +430,9 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
                                       s.bx, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "stage kernel launch");
     if (ev) JX_HIP(c, hipEventRecord(ev->second, c->stream));
+    if (last && a.u_out != A)
+      JX_HIP(c, hipMemcpyAsync(A, a.u_out, (size_t)p->nx * p->ny * p->nvar * sizeof(T),
+                               hipMemcpyDeviceToDevice, c->stream));
   }
   if (raw) {
     // Kick: forcing.py:137-149 then hydro.py:132, on the post-drag state in A.
@@ -680,6 +702,19 @@ int viscous_impl(jx_ctx* c, int ny, int nx, int nv, const T* W, double dx, doubl
   const jx::Over<T, false> o4dy{T(4.0 * dy), T(1.0 / (4.0 * dy))};
   hipLaunchKernelGGL(jx::viscous_faces_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, W, out,
                      nx, ny, nv, axis, odx, ody, o4dx, o4dy, vp);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
+}
+
+template <typename T>
+int gradients_impl(jx_ctx* c, int ny, int nx, int nv, const T* W, double dx, double dy,
+                   int axis, T* dvdx, T* dvdy) {
+  const dim3 g = aux_grid((size_t)nx * ny);
+  const jx::Over<T, false> odx{T(dx), T(1.0 / dx)}, ody{T(dy), T(1.0 / dy)};
+  const jx::Over<T, false> o4dx{T(4.0 * dx), T(1.0 / (4.0 * dx))};
+  const jx::Over<T, false> o4dy{T(4.0 * dy), T(1.0 / (4.0 * dy))};
+  hipLaunchKernelGGL(jx::face_gradients_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, W,
+                     dvdx, dvdy, nx, ny, nv, axis, odx, ody, o4dx, o4dy);
   JX_HIP(c, hipGetLastError());
   return JX_OK;
 }
@@ -1066,6 +1101,40 @@ int jx_viscous_momentum_flux(jx_ctx* c, int dtype, int ny, int nx, int nvar, con
                                 viscosity, smagorinsky_cs, axis, static_cast<double*>(out));
   return viscous_impl<float>(c, ny, nx, nvar, static_cast<const float*>(W), dx, dy, viscosity,
                              smagorinsky_cs, axis, static_cast<float*>(out));
+}
+
+
+int jx_face_velocity_gradients(jx_ctx* c, int dtype, int ny, int nx, int nvar, const void* W,
+                               double dx, double dy, int axis, void* dv_dx, void* dv_dy) {
+  if (!c) return JX_EINVAL;
+  if (ny < 1 || nx < 1 || nvar < 3 || (axis != 0 && axis != 1) || !W || !dv_dx || !dv_dy)
+    return fail(c, JX_EINVAL, "bad face_velocity_gradients arguments");
+  if (dtype == 0)
+    return gradients_impl<double>(c, ny, nx, nvar, static_cast<const double*>(W), dx, dy, axis,
+                                  static_cast<double*>(dv_dx), static_cast<double*>(dv_dy));
+  return gradients_impl<float>(c, ny, nx, nvar, static_cast<const float*>(W), dx, dy, axis,
+                               static_cast<float*>(dv_dx), static_cast<float*>(dv_dy));
+}
+
+int jx_smagorinsky_viscosity(jx_ctx* c, int dtype, long long n, const void* dv_dx,
+                             const void* dv_dy, double dx, double dy, double smagorinsky_cs,
+                             void* nu) {
+  if (!c) return JX_EINVAL;
+  if (n < 0 || !dv_dx || !dv_dy || !nu) return fail(c, JX_EINVAL, "bad smagorinsky arguments");
+  if (n == 0) return JX_OK;
+  // (Cs**2 * dx) * dy in float64, then times the strain norm (fluxes.py:245).
+  const double k = smagorinsky_cs * smagorinsky_cs * dx * dy;
+  const dim3 g = aux_grid((size_t)n);
+  if (dtype == 0)
+    hipLaunchKernelGGL(jx::smagorinsky_kernel<double>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const double*>(dv_dx), static_cast<const double*>(dv_dy),
+                       static_cast<double*>(nu), (size_t)n, k);
+  else
+    hipLaunchKernelGGL(jx::smagorinsky_kernel<float>, g, dim3(jx::kAuxBlock), 0, c->stream,
+                       static_cast<const float*>(dv_dx), static_cast<const float*>(dv_dy),
+                       static_cast<float*>(nu), (size_t)n, (float)k);
+  JX_HIP(c, hipGetLastError());
+  return JX_OK;
 }
 
 int jx_add_momentum_flux(jx_ctx* c, int dtype, long long ncells, int nvar, void* flux,

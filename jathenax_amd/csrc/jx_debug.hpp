@@ -26,7 +26,8 @@ __device__ __forceinline__ size_t along(int j, int i, int o, int axis, int nx, i
 }
 
 // reconstruct_faces (fluxes.py:124-131): left = high edge of the cell below the face,
-// right = low edge of the cell above it; PPMX and WENO-Z floor the face density.
+// right = low edge of the cell above it; PPMX and WENO-Z floor the face density unless
+// dfloor is NaN (the registry functions *_face_states, fluxes.py:6-117, do not floor).
 template <typename T, int RECON>
 __global__ void __launch_bounds__(kAuxBlock) face_states_kernel(const T* W, T* left, T* right,
                                                                 int nx, int ny, int nv,
@@ -46,7 +47,7 @@ __global__ void __launch_bounds__(kAuxBlock) face_states_kernel(const T* W, T* l
       cell_edges<RECON>(q[0], q[1], q[2], q[3], q[4], eps, lo_b, hi_b);   // cell below
       cell_edges<RECON>(q[1], q[2], q[3], q[4], q[5], eps, lo_c, hi_c);   // this cell
       T l = hi_b, r = lo_c;
-      if (floors_faces<RECON>() && v == 0) {
+      if (floors_faces<RECON>() && v == 0 && dfloor == dfloor) {
         l = fmx(l, dfloor);
         r = fmx(r, dfloor);
       }
@@ -120,6 +121,47 @@ __global__ void __launch_bounds__(kAuxBlock) viscous_faces_kernel(const T* W, T*
     out[c * 2 + 0] = fmx_;
     out[c * 2 + 1] = fmy_;
   }
+}
+
+// face_velocity_gradients (fluxes.py:227-238): (dv/dx, dv/dy), each (ny, nx, 2), at the
+// low face of each cell along `axis`: the normal difference of the two cells at the face
+// and the average of their centred transverse differences, in the expression order of
+// viscous_faces_kernel (so the gradients are the ones the viscous flux uses).
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) face_gradients_kernel(const T* W, T* dvdx, T* dvdy,
+                                                                   int nx, int ny, int nv,
+                                                                   int axis,
+                                                                   Over<T, false> over_dx,
+                                                                   Over<T, false> over_dy,
+                                                                   Over<T, false> over_4dx,
+                                                                   Over<T, false> over_4dy) {
+  const size_t n = (size_t)nx * ny;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(c / nx), i = (int)(c % nx);
+    const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
+    const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+    auto w = [&](int jj, int ii, int v) { return W[((size_t)jj * nx + ii) * nv + v]; };
+    for (int v = 1; v <= 2; ++v) {
+      if (axis == 1) {
+        dvdx[c * 2 + v - 1] = over_dx(w(j, i, v) - w(j, im, v));
+        dvdy[c * 2 + v - 1] = over_4dy((w(jp, i, v) + w(jp, im, v)) - (w(jm, i, v) + w(jm, im, v)));
+      } else {
+        dvdy[c * 2 + v - 1] = over_dy(w(j, i, v) - w(jm, i, v));
+        dvdx[c * 2 + v - 1] = over_4dx((w(j, ip, v) + w(jm, ip, v)) - (w(j, im, v) + w(jm, im, v)));
+      }
+    }
+  }
+}
+
+// smagorinsky_viscosity (fluxes.py:241-245) from face gradients: c sqrt(2 (ux^2 + vy^2) +
+// (uy + vx)^2) with c = Cs^2 dx dy rounded once on the host, as smag_nu in the fused stage.
+template <typename T>
+__global__ void __launch_bounds__(kAuxBlock) smagorinsky_kernel(const T* dvdx, const T* dvdy,
+                                                                T* nu, size_t n, T c) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    nu[k] = smag_nu(dvdx[2 * k], dvdy[2 * k + 1], dvdx[2 * k + 1] + dvdy[2 * k], c);
 }
 
 // flux[..., 1:3] += add (the viscous and learned-stress additions, hydro.py:106-119).

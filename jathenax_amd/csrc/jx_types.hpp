@@ -33,26 +33,39 @@ struct Ctrl {
   double pad1[2];
 };
 
-// Runtime parameters of one isothermal run (the static arguments of
-// evolve_isothermal, run.py:24-27, plus grid geometry).
-struct RunParams {
-  int nx, ny, nvar;
-  int recon, rsolver;
-  int integrator;            // number of RK stages: 2, 3 or 4
-  double dx, dy, cs, dfloor, cfl;
-  double viscosity, smag_cs, drag_rate;
-  double tlim;
-  int nlim;
-  int has_forcing;
-  double dt_update, tdriv_start, dedt;
-  int n_updates, n_modes;
-};
-
 // One RK stage's coefficient row (hydro.py:20-36) resolved on the host.
 struct StageCoef {
   double gam0, gam1, beta, delta;
   int has_delta;   // RK4 stages 2-4 accumulate the saved register
   int blended;     // gam0 != 0 and gam0 + gam1 == 1.0 in float64 (hydro.py:46)
 };
+
+
+// Built-in RK rows (hydro.py:20-36) of rk2, rk3 and rk4 by stage count - 2:
+// (gam0, gam1, beta, delta) and whether the row has a delta (None otherwise).
+struct RkRow { double g0, g1, beta, delta; int has_delta; };
+constexpr RkRow kBuiltinRk[3][4] = {
+    {{0.0, 1.0, 1.0, 0.0, 0}, {0.5, 0.5, 0.5, 0.0, 0}, {}, {}},
+    {{0.0, 1.0, 1.0, 0.0, 0}, {0.25, 0.75, 0.25, 0.0, 0}, {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0.0, 0},
+     {}},
+    {{0.0, 1.0, 1.193743905974738, 0.0, 0},
+     {0.121098479554482, 0.721781678111411, 0.099279895495783, 0.217683334308543, 1},
+     {-3.843833699660025, 2.121209265338722, 1.131678018054042, 1.065841341361089, 1},
+     {0.546370891121863, 0.198653035682705, 0.310665766509336, 0.0, 1}}};
+
+// A caller's table (rows of gam0, gam1, beta, delta with delta NaN for None) equals the
+// built-in rows of n stages.
+inline bool rk_rows_are_builtin(const double* rows, int n) {
+  if (n < 2 || n > 4) return false;
+  for (int k = 0; k < n; ++k) {
+    const RkRow& b = kBuiltinRk[n - 2][k];
+    const double* r = rows + 4 * k;
+    const bool none = r[3] != r[3];
+    if (r[0] != b.g0 || r[1] != b.g1 || r[2] != b.beta || none == (b.has_delta != 0) ||
+        (!none && r[3] != b.delta))
+      return false;
+  }
+  return true;
+}
 
 }  // namespace jx

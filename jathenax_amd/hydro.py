@@ -11,7 +11,7 @@ import torch
 
 from . import _lib
 from .device import as_device, context_for, dtype_id, ptr
-from .fluxes import RECONSTRUCTIONS, RIEMANN_SOLVERS
+from .fluxes import fused_operator_ids
 
 # (gam0, gam1, beta, delta) rows; the table is part of the interface (hydro.py:20-36).
 RK_STAGES = {
@@ -23,28 +23,51 @@ RK_STAGES = {
             (-3.843833699660025, 2.121209265338722, 1.131678018054042, 1.065841341361089),
             (0.546370891121863, 0.198653035682705, 0.310665766509336, 0.0)),
 }
-_N_STAGES = {"rk2": 2, "rk3": 3, "rk4": 4}
 MAX_NVAR = 5
+MAX_RK_STAGES = 8
+
+
+def rk_table(integrator):
+    """The integrator's rows of RK_STAGES as a flat float64 ctypes array, looked up on every
+    call as hydro.py:100 does (KeyError for an unknown name); delta None becomes NaN."""
+    rows = RK_STAGES[integrator]
+    if not 1 <= len(rows) <= MAX_RK_STAGES:
+        raise ValueError(f"RK_STAGES[{integrator!r}] must have 1 to {MAX_RK_STAGES} rows")
+    flat = []
+    for row in rows:
+        gam0, gam1, beta, delta = row
+        flat += [float(gam0), float(gam1), float(beta),
+                 math.nan if delta is None else float(delta)]
+    return len(rows), (ctypes.c_double * len(flat))(*flat)
 
 
 def run_params(U, dx, dy, cs, dfloor, integrator, *, reconstruct="ppm4", rsolver="roe",
                viscosity=0.0, smagorinsky_cs=0.0, drag_rate=0.0, cfl=0.0, tlim=0.0, nlim=0,
                dt_update=1.0, tdriv_start=0.0, dedt=0.0):
-    """Validate options like the reference (KeyError for unknown names) and pack them."""
-    stages = _N_STAGES[integrator] if integrator in _N_STAGES else RK_STAGES[integrator]
-    recon = RECONSTRUCTIONS[reconstruct]
-    rs = RIEMANN_SOLVERS[rsolver]
+    """Validate options like the reference (KeyError for unknown names) and pack them.
+    The RK rows travel with the parameters, so an edited or added RK_STAGES entry is
+    applied as the reference applies it."""
+    stages, rows = rk_table(integrator)
+    ids = fused_operator_ids(reconstruct, rsolver)
+    if ids is None:
+        raise ValueError(f"RECONSTRUCTIONS[{reconstruct!r}] or RIEMANN_SOLVERS[{rsolver!r}] is "
+                         "not a built-in operator; only isothermal_step and evolve_isothermal "
+                         "run replaced operators (on the unfused stage path)")
+    recon, rs = ids
     if U.ndim != 3:
         raise ValueError("state must have shape (ny, nx, nvar)")
     ny, nx, nvar = U.shape
     if not 3 <= nvar <= MAX_NVAR:
         raise ValueError(f"nvar must be in [3, {MAX_NVAR}] (hydro plus up to two scalars)")
-    return _lib.RunParams(
+    p = _lib.RunParams(
         ny=ny, nx=nx, nvar=nvar, dtype=dtype_id(U), reconstruct=recon, rsolver=rs,
         integrator=stages, nlim=int(nlim), dx=float(dx), dy=float(dy), cs=float(cs),
         dfloor=float(dfloor), cfl=float(cfl), tlim=float(tlim), viscosity=float(viscosity),
         smagorinsky_cs=float(smagorinsky_cs), drag_rate=float(drag_rate),
-        dt_update=float(dt_update), tdriv_start=float(tdriv_start), dedt=float(dedt))
+        dt_update=float(dt_update), tdriv_start=float(tdriv_start), dedt=float(dedt),
+        rk_rows=ctypes.cast(rows, ctypes.POINTER(ctypes.c_double)))
+    p._rk_rows = rows   # keeps the table alive as long as the parameters
+    return p
 
 
 def rk_stage(U, U_saved, div_flux, dt, coefficients):
@@ -103,10 +126,12 @@ def isothermal_step(U, dt, dx, dy, cs, dfloor, integrator, *,
     """Advance one periodic isothermal step; every RK stage is one fused HIP launch.
 
     trace=True runs the stages as separate launches and returns (U, stages) with the
-    reference's dictionaries.  predict_stress(sgs_weights, primitives) is called before
+    reference's dictionaries; a registry entry replaced by the caller's own operator also
+    takes that path.  predict_stress(sgs_weights, primitives) is called before
     each stage with the stage's (rho, vx, vy) as a (ny, nx, 3) tensor view and must return
     a floating (ny, nx, 3) stress, which the fused stage adds to the momentum fluxes."""
-    if trace:
+    RK_STAGES[integrator]
+    if trace or fused_operator_ids(reconstruct, rsolver) is None:
         from .trace import traced_step
         return traced_step(U, dt, dx, dy, cs, dfloor, integrator, viscosity=viscosity,
                            smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate,

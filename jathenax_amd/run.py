@@ -17,7 +17,8 @@ import torch
 
 from . import _lib, tuning
 from .device import as_device, context_for, dtype_id, ptr
-from .hydro import run_params
+from .fluxes import fused_operator_ids
+from .hydro import RK_STAGES, run_params
 
 
 def _evolve_raw(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, **kw):
@@ -36,6 +37,16 @@ def evolve_isothermal(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, *,
             raise ValueError(f"{name} must be finite and nonnegative")
     if forcing is not None and (not 0.0 < dt_update < np.inf or not math.isfinite(tdriv_start)):
         raise ValueError("dt_update must be finite and positive and tdriv_start finite")
+    RK_STAGES[integrator]
+    if fused_operator_ids(reconstruct, rsolver) is None:
+        # A replaced registry operator: the loop on the host over unfused-path steps.
+        from .trace import evolve_with_predictor
+        return evolve_with_predictor(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim,
+                                     viscosity=viscosity, smagorinsky_cs=smagorinsky_cs,
+                                     drag_rate=drag_rate, forcing=forcing, dt_update=dt_update,
+                                     tdriv_start=tdriv_start, dedt=dedt,
+                                     reconstruct=reconstruct, rsolver=rsolver,
+                                     predict_stress=predict_stress, sgs_weights=sgs_weights)
     return _evolve(U, dx, dy, cs, dfloor, integrator, cfl, tlim, nlim, viscosity=viscosity,
                    smagorinsky_cs=smagorinsky_cs, drag_rate=drag_rate, forcing=forcing,
                    dt_update=dt_update, tdriv_start=tdriv_start, dedt=dedt,

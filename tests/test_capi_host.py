@@ -34,9 +34,10 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_run_params_layout_matches_header():
-    # 8 ints then 12 doubles, no padding surprises.
-    assert ctypes.sizeof(_lib.RunParams) == 8 * 4 + 12 * 8
+    # 8 ints, 12 doubles, then the RK-table pointer; no padding surprises.
+    assert ctypes.sizeof(_lib.RunParams) == 8 * 4 + 12 * 8 + 8
     assert _lib.RunParams.dx.offset == 32
+    assert _lib.RunParams.rk_rows.offset == 8 * 4 + 12 * 8
 
 
 def test_seed_streams_match_reference(lib, ref_vectors):

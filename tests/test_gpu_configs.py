@@ -1,0 +1,133 @@
+"""The benchmark workloads at their own definitions, against the oracle where it runs in
+seconds and through size-independent properties where it does not.
+
+* plain method: the smooth wave on 64 x 48, PPM4 + Roe, RK3, CFL 0.4, to t = 0.2 and for
+  100 cycles, FP64 against the oracle (relative L-infinity <= 1e-12), mass and momentum
+  conserved to round-off;
+* Kelvin-Helmholtz with a tracer: tanh interfaces, nvar 4, PPM4 + Roe, RK3, CFL 0.4 on 64^2
+  for 120 cycles, FP64 against the oracle, mass, momentum and tracer conserved, and the
+  FP32 run against FP64 by L1 of density and tracer;
+* the 4096^2 headline grid: one FP64 RK3 step of the device loop at the tile shape the
+  benchmark uses against the oracle (<= 1e-13), and FP32 against FP64 after five steps;
+* serving: requests drawn by the serving workload generator at its own size and end-time
+  ranges (square grids 128..2048, tlim U[0.1, 2]), each result served in the mix equal bit
+  for bit to the same request served alone.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import jathenax_np as ox
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def jx():
+    import jathenax_amd
+    assert torch.cuda.is_available(), "GPU test needs a visible ROCm device"
+    return jathenax_amd
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel_inf(a, b):
+    return float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
+
+
+def _l1(a, b):
+    return float(np.mean(np.abs(np.asarray(a, dtype=np.float64) - b)))
+
+
+@pytest.mark.parametrize("tlim,nlim", [(0.2, 100000), (10.0, 100)])
+def test_plain_method_smooth_wave(jx, tlim, nlim):
+    from jathenax_amd import problems
+    U0 = problems.smooth_wave(64, 48)
+    args = (1 / 64, 1 / 48, 1.0, 1e-10, "rk3", 0.4, tlim, nlim)
+    U, t, dt, cyc = jx.evolve_isothermal(torch.tensor(U0, device="cuda"), *args)
+    Uo, to, dto, co = ox.evolve_isothermal(U0, *args)
+    assert cyc == co and (cyc == 100 if nlim == 100 else t == tlim == to)
+    assert _rel_inf(_np(U), Uo) <= 1e-12
+    assert abs(t - to) <= 1e-12 * to and abs(dt - dto) <= 1e-12 * dto
+    s0, s1 = ox.conserved_sums(U0), jx.conserved_sums(U)
+    assert np.max(np.abs(s1 - s0)) <= 1e-12 * U0[..., 0].size
+
+
+# FP32 against FP64 for the KH workload after 120 cycles (t ~ 0.5), stated bounds: the
+# instability grows round-off, so the tracer (a sharp front) sits above the density.
+KH_FP32_L1 = {"density": 1e-5, "tracer": 1e-4}
+
+
+def test_kelvin_helmholtz_tracer_workload(jx):
+    from jathenax_amd import problems
+    n = 64
+    U0 = problems.kelvin_helmholtz_tanh(n, n)
+    args = (1 / n, 1 / n, 1.0, 1e-10, "rk3", 0.4, 3.0, 120)
+    U, t, dt, cyc = jx.evolve_isothermal(torch.tensor(U0, device="cuda"), *args)
+    Uo, to, dto, co = ox.evolve_isothermal(U0, *args)
+    assert cyc == co == 120
+    U64 = _np(U)
+    assert _rel_inf(U64, Uo) <= 1e-12
+    assert abs(t - to) <= 1e-12 * to
+    # mass, both momenta and the tracer mass to round-off
+    s0, s1 = ox.conserved_sums(U0), jx.conserved_sums(U)
+    assert s1.shape == (4,)
+    assert np.max(np.abs(s1 - s0)) <= 1e-12 * n * n
+    assert np.max(np.abs(U64[..., 3] - U0[..., 3])) > 1e-3   # the tracer moved
+    U32, t32, _, c32 = jx.evolve_isothermal(torch.tensor(U0, dtype=torch.float32,
+                                                         device="cuda"), *args)
+    assert c32 == 120
+    U32 = _np(U32)
+    l1 = {"density": _l1(U32[..., 0], U64[..., 0]),
+          "tracer": _l1(U32[..., 3] / U32[..., 0], U64[..., 3] / U64[..., 0])}
+    print(f"KH 64^2 FP32 vs FP64 after 120 cycles: {l1}")
+    for k, bound in KH_FP32_L1.items():
+        assert l1[k] <= bound, (k, l1[k])
+
+
+def test_headline_grid_step_against_the_oracle(jx):
+    """4096^2 smooth wave, PPM4 + Roe, RK3: one step of the device loop (CFL dt, fused stages
+    at the tiles the benchmark takes from the tile table) against the oracle."""
+    from jathenax_amd import problems
+    from jathenax_amd.device import context_for
+    from jathenax_amd.hydro import run_params
+    n = 4096
+    U0 = problems.smooth_wave(n, n)
+    args = (1 / n, 1 / n, 1.0, 1e-10, "rk3", 0.4, 10.0, 1)
+    Ud = torch.tensor(U0, device="cuda")
+    U, t, dt, cyc = jx.evolve_isothermal(Ud, *args)
+    p = run_params(Ud, 1 / n, 1 / n, 1.0, 1e-10, "rk3")
+    print(f"4096^2 FP64 step at tiles {context_for(Ud).tiles_for(p)}", flush=True)
+    Uo, to, dto, co = ox.evolve_isothermal(U0, *args)
+    assert cyc == co == 1 and dt == dto
+    assert _rel_inf(_np(U), Uo) <= 1e-13
+    del Uo
+    # FP32 after five steps against FP64 after five steps
+    args5 = args[:-1] + (5,)
+    U64 = _np(jx.evolve_isothermal(Ud, *args5)[0])[..., 0]
+    U32 = _np(jx.evolve_isothermal(Ud.float(), *args5)[0])[..., 0]
+    l1 = _l1(U32, U64)
+    print(f"4096^2 FP32 vs FP64 after 5 steps: L1(rho) {l1:.3e}")
+    assert l1 <= 1e-6
+
+
+def test_serving_workload_alone_equals_in_mix():
+    from jathenax_amd import workloads
+    from jathenax_amd.serving import Request, Server
+    # seed 3: grids 129^2 .. 1814^2 (the largest on the single-grid path), tlim 0.4 .. 1.95
+    reqs = [Request(rid=k, spec=spec, arrival=t)
+            for k, (t, spec) in enumerate(workloads.serving_requests(n=6, rate=20.0, seed=3))]
+    sides = sorted(r.spec["U"].shape[0] for r in reqs)
+    assert sides[0] < 192 and sides[-1] > 1536
+    mix = Server(pool_bytes=1 << 30, max_instances=16).serve(reqs)
+    assert all(r.status == "done" for r in mix), [(r.rid, r.status, r.info) for r in mix]
+    assert {r.path for r in mix} == {"grid", "batch"}
+    for r in mix:
+        alone = Server(pool_bytes=1 << 30, max_instances=2).serve(
+            [Request(rid=r.rid, spec=r.spec, arrival=0.0)])[0]
+        assert alone.status == "done" and alone.path == r.path
+        np.testing.assert_array_equal(alone.result[0].cpu().numpy(), r.result[0].cpu().numpy())
+        assert alone.result[1:] == r.result[1:]
