@@ -265,3 +265,47 @@ def test_edited_rk_row_is_applied(monkeypatch):
         Ensemble(torch.float64).add(U, 1 / 22, 1 / 18, 1.0, 1e-10, "rk3", 0.4, 0.01, 2)
     with pytest.raises(KeyError):
         jx.isothermal_step(_dev(U), 2e-3, 1 / 22, 1 / 18, 1.0, 1e-10, "rk5")
+
+
+# ---- FP32 against FP64 for every reconstruction x solver pair --------------------------
+
+
+def _hard_flow(n):
+    """Supersonic x drift over a density wave, with a velocity jump of 5 cs at x = 0 (a
+    vacuum-forming rarefaction, where Roe falls back to LLF on the first steps) and -5 cs at
+    the periodic seam (a shock)."""
+    x = (np.arange(n) + 0.5) / n - 0.5
+    X, Y = np.meshgrid(x, x)
+    rho = 1.0 + 0.5 * np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+    vx = 2.0 + 0.5 * np.cos(2 * np.pi * Y) + 2.5 * np.sign(X)
+    vy = 0.8 * np.sin(2 * np.pi * X)
+    return np.stack((rho, rho * vx, rho * vy), axis=-1)
+
+
+# Relative L1 of FP32 against FP64 after 10 RK3 steps of the hard flow, stated bound.
+PAIR_FP32_REL_L1 = 1e-5
+
+
+@pytest.mark.parametrize("recon", RECONS)
+@pytest.mark.parametrize("rs", SOLVERS)
+def test_fp32_every_operator_pair_tracks_fp64(recon, rs):
+    import jathenax_amd as jx
+    n = 64
+    U0 = _hard_flow(n)
+    args = (1 / n, 1 / n, 1.0, 1e-10, "rk3", 0.3, 10.0, 10)
+    kw = dict(reconstruct=recon, rsolver=rs)
+    # the flow exercises the branches the FP32 forms rewrite
+    _, W = ox.recover_isothermal(U0, 1e-10)
+    L, R = ox.reconstruct_faces(W, 1, recon, 1e-10)
+    assert (L[..., 1] - 1.0 > 0).mean() > 0.5            # supersonic x faces
+    assert (R[..., 1] - L[..., 1] > 2.0).sum() >= n      # Roe's LLF fallback faces
+    U64, t64, dt64, c64 = jx.evolve_isothermal(_dev(U0), *args, **kw)
+    Uo = ox.evolve_isothermal(U0, *args, **kw)[0]
+    U64 = _np(U64)
+    assert np.max(np.abs(U64 - Uo)) <= 1e-12 * np.max(np.abs(Uo))
+    U32, t32, dt32, c32 = jx.evolve_isothermal(_dev(U0, torch.float32), *args, **kw)
+    assert c32 == c64 == 10
+    rel = float(np.sum(np.abs(_np(U32).astype(np.float64) - U64)[..., 0])
+                / np.sum(np.abs(U64[..., 0])))
+    print(f"FP32 vs FP64 {recon}+{rs}, 10 RK3 steps: relative L1(rho) {rel:.3e}")
+    assert rel <= PAIR_FP32_REL_L1
