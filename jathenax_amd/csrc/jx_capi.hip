@@ -350,23 +350,48 @@ template <typename T> StepPlan<T> make_plan(jx_ctx* c, const jx_run_params* p,
   return s;
 }
 
+// Device memory a step uses beyond its input and output: the stage registers B, C and the
+// RK4 saved register S (each one state), and the per-block partial sums of the kick.
+// `timing`: bracket the stage launches with the context's HIP events.
+template <typename T> struct StepScratch {
+  T* B;
+  T* C;
+  T* S;
+  double* partial;
+  bool timing;
+};
+
+// Partial sums a step with the kick needs: the stage's per-block moments (at most
+// kick_sums per block) and the second-moment pass (3 per aux block, at most 4096 blocks).
+size_t kick_partials(size_t nblocks) { return std::max<size_t>(nblocks, 4096) * 7; }
+
+// The context's own scratch for a step, allocated (grown) on demand.
+template <typename T>
+int ctx_scratch(jx_ctx* c, size_t state_bytes, size_t nblocks, bool kick, StepScratch<T>* out) {
+  int rc = ensure_buffers(c, state_bytes);
+  if (rc) return rc;
+  if (kick) {
+    rc = ensure_partial(c, kick_partials(nblocks));
+    if (rc) return rc;
+  }
+  *out = StepScratch<T>{static_cast<T*>(c->buf[0]), static_cast<T*>(c->buf[1]),
+                        static_cast<T*>(c->buf[2]), c->partial, c->timing};
+  return JX_OK;
+}
+
 // Launch the RK stages of one step.  `first`: the saved register is the caller's input
 // (floored on read).  The result lands in `A`.  `raw`: kick the final stage.
 template <typename T>
-int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
-                const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr) {
+int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const T* in, T* A,
+                bool first, const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr) {
   const jx_run_params* p = s.p;
   const RkTable tab = rk_table(p);
   const int ns = tab.n;
-  T* B = static_cast<T*>(c->buf[0]);
-  T* C = static_cast<T*>(c->buf[1]);
-  T* S = static_cast<T*>(c->buf[2]);
+  T* B = w.B;
+  T* C = w.C;
+  T* S = w.S;
   const T* src = first ? in : A;
   const size_t nblocks = (size_t)s.grid.x * s.grid.y;
-  if (raw) {
-    int rc = ensure_partial(c, std::max<size_t>(nblocks, 4096) * jx::kick_sums<T, true>());
-    if (rc) return rc;
-  }
   // Register plan.  Stage outputs alternate B, C, B, ... and the last goes to A; a stage
   // never reads its stencil from the buffer it writes.  The saved register is the step's
   // input (floored on read when that is the caller's array) until a row with a delta
@@ -400,7 +425,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
     a.dtkey = nullptr;
     if (last && raw) {
       a.raw = raw;
-      a.partial = c->partial;
+      a.partial = w.partial;
     } else if (last && want_dtkey) {
       a.dtkey = &c->ctrl->dtkey;
     }
@@ -416,7 +441,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
       a.tau = hook->tau;
     }
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-    if (c->timing) {
+    if (w.timing) {
       if (c->events_used == c->events.size()) {
         std::pair<hipEvent_t, hipEvent_t> pr{};
         JX_HIP(c, hipEventCreate(&pr.first));
@@ -439,7 +464,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
     const size_t n = (size_t)p->nx * p->ny;
     const bool one_pass = s.visc && jx::kick_sums<T, true>() == 7;   // as the stage kernel
     hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
-                       c->partial, (int)nblocks, one_pass ? 2 : 0, (double)n);
+                       w.partial, (int)nblocks, one_pass ? 2 : 0, (double)n);
     jx::KickArgs<T> ka{};
     ka.u = A;
     ka.raw = raw;
@@ -451,13 +476,13 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const T* in, T* A, bool first,
     ka.dy = T(p->dy);
     ka.cs = T(p->cs);
     ka.ctrl = c->ctrl;
-    ka.partial = c->partial;
+    ka.partial = w.partial;
     const dim3 g = aux_grid(n);
     if (!one_pass) {
       hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0,
                          c->stream, ka);
       hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
-                         c->partial, (int)g.x, 1, (double)n);
+                         w.partial, (int)g.x, 1, (double)n);
     }
     ka.dtkey = want_dtkey ? &c->ctrl->dtkey : nullptr;
     hipLaunchKernelGGL(jx::kick_apply_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
@@ -493,11 +518,12 @@ template <typename T>
 int step_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, double dt, const T* raw,
               double dedt, const SgsHook<T>* hook = nullptr) {
   const size_t bytes = (size_t)p->nx * p->ny * p->nvar * sizeof(T);
-  int rc = ensure_buffers(c, bytes);
-  if (rc) return rc;
   StepPlan<T> s = make_plan<T>(c, p, hook != nullptr);
+  StepScratch<T> w{};
+  int rc = ctx_scratch<T>(c, bytes, (size_t)s.grid.x * s.grid.y, raw != nullptr, &w);
+  if (rc) return rc;
   hipLaunchKernelGGL(jx::set_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, dt, dedt);
-  return launch_step<T>(c, s, in, out, true, raw, false, hook);
+  return launch_step<T>(c, s, w, in, out, true, raw, false, hook);
 }
 
 template <typename T>
@@ -506,9 +532,11 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
                 const jx_loop_state* start = nullptr, jx_loop_state* end = nullptr) {
   const size_t n = (size_t)p->nx * p->ny;
   const size_t bytes = n * p->nvar * sizeof(T);
-  int rc = ensure_buffers(c, bytes);
-  if (rc) return rc;
   const bool forced = f != nullptr;
+  StepPlan<T> s = make_plan<T>(c, p, hook != nullptr);
+  StepScratch<T> w{};
+  int rc = ctx_scratch<T>(c, bytes, (size_t)s.grid.x * s.grid.y, forced, &w);
+  if (rc) return rc;
   if (forced) {
     rc = ensure(c, &c->raw, &c->raw_bytes, n * 2 * sizeof(T));
     if (rc) return rc;
@@ -528,7 +556,6 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     if (end) *end = s0;
     return JX_OK;
   }
-  StepPlan<T> s = make_plan<T>(c, p, hook != nullptr);
   const bool smag = p->smagorinsky_cs > 0.0;
   jx::LoopArgs la{};
   la.tlim = p->tlim;
@@ -574,7 +601,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
   auto one_step = [&](bool first) -> int {
     hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
     if (forced) launch_render<T>(c, ra);
-    int r = launch_step<T>(c, s, in, out, first, raw, !smag, hook);
+    int r = launch_step<T>(c, s, w, in, out, first, raw, !smag, hook);
     if (r) return r;
     if (smag)
       hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
@@ -607,7 +634,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     chunk = hook ? 1 : std::max<long long>(1, std::min<long long>(want, 512));
     {
       if (forced) launch_render<T>(c, ra);
-      int r = launch_step<T>(c, s, in, out, false, raw, !smag, hook);
+      int r = launch_step<T>(c, s, w, in, out, false, raw, !smag, hook);
       if (r) return r;
       if (smag)
         hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny),
