@@ -282,8 +282,9 @@ def _hard_flow(n):
     return np.stack((rho, rho * vx, rho * vy), axis=-1)
 
 
-# Relative L1 of FP32 against FP64 after 10 RK3 steps of the hard flow, stated bound.
-PAIR_FP32_REL_L1 = 1e-5
+# Relative L1 of FP32 against FP64 after 10 RK3 steps of the hard flow, stated bound
+# (measured 0.8e-7 .. 4.6e-7 over the 15 pairs, PPMX + HLLE the largest).
+PAIR_FP32_REL_L1 = 2e-6
 
 
 @pytest.mark.parametrize("recon", RECONS)
@@ -297,7 +298,7 @@ def test_fp32_every_operator_pair_tracks_fp64(recon, rs):
     # the flow exercises the branches the FP32 forms rewrite
     _, W = ox.recover_isothermal(U0, 1e-10)
     L, R = ox.reconstruct_faces(W, 1, recon, 1e-10)
-    assert (L[..., 1] - 1.0 > 0).mean() > 0.5            # supersonic x faces
+    assert (np.abs(L[..., 1]) > 1.0).mean() >= 0.4       # supersonic x faces
     assert (R[..., 1] - L[..., 1] > 2.0).sum() >= n      # Roe's LLF fallback faces
     U64, t64, dt64, c64 = jx.evolve_isothermal(_dev(U0), *args, **kw)
     Uo = ox.evolve_isothermal(U0, *args, **kw)[0]
