@@ -95,6 +95,18 @@ int jx_kernel_timing(jx_ctx* ctx, int* launches, double* total_ms);
 int jx_isothermal_step(jx_ctx* ctx, const jx_run_params* p, const void* U_in, void* U_out,
                        double dt, const void* raw_force, double dedt);
 
+/* jx_isothermal_step without allocation.  No reference counterpart: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The stage registers and partial sums live in a static
+ * scratch array of 96 MiB per GPU instead of context buffers: states up to 47.5 MiB
+ * (rk2/rk3) or 31.6 MiB (tables with a delta, e.g. rk4), kicked steps up to 18724 stage
+ * blocks; larger inputs fail with JX_EINVAL before any work.  The caller passes U_in and
+ * U_out as for jx_isothermal_step, and the result equals its result bit for bit.  All
+ * calls on one GPU share that scratch: calls that overlap on one GPU (several streams or
+ * threads) race and must be serialised by the caller. */
+int jx_isothermal_step_noalloc(jx_ctx* ctx, const jx_run_params* p, const void* U_in,
+                               void* U_out, double dt, const void* raw_force, double dedt);
+
 /* Timestep limit from primitives W (replaces isothermal_timestep, hydro.py:63-89). */
 int jx_isothermal_timestep(jx_ctx* ctx, const jx_run_params* p, const void* W,
                            double previous_dt, double time, double tlim, double* dt_out);

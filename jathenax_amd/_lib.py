@@ -49,6 +49,8 @@ SIGNATURES = {
     "jx_kernel_timing": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double)]),
     "jx_isothermal_step": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p, c_double,
                                    c_void_p, c_double]),
+    "jx_isothermal_step_noalloc": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,
+                                           c_double, c_void_p, c_double]),
     "jx_isothermal_step_sgs": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p,
                                        c_double, c_void_p, c_double, c_void_p, c_void_p,
                                        STRESS_FN, c_void_p]),

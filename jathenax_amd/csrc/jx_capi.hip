@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -25,6 +26,16 @@ struct jx_batch {
   jx::BatchBase* impl;
   std::string err;
 };
+
+// Static scratch of jx_isothermal_step_noalloc: device memory that needs no allocation at
+// call time.  Each GPU holds its own copy until the process ends.  Layout: the kick's
+// per-block partial sums (the first kNoAllocPartialBytes), then the stage registers B, C
+// and, for RK tables with a delta, S, each rounded up to 256 bytes.  Host code never uses
+// the array's name as an address: jx_ctx_create takes each device's address once with
+// hipGetSymbolAddress (scratch_setup), and only that address reaches kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)96 << 20;
+constexpr size_t kNoAllocPartialBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 using jx::Ctrl;
 using jx::StageArgs;
@@ -71,6 +82,37 @@ struct jx_ctx {
 };
 
 namespace {
+
+// Per-device address of `scratch` (nullptr until set up) and the setup's status.
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kScratchDevices] = {};
+hipError_t g_scratch_status[kScratchDevices] = {};
+
+// Once per device, with that device current: the device address of `scratch`.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && p) {
+    g_scratch_addr[device] = p;
+  } else {
+    g_scratch_status[device] = (e == hipSuccess) ? hipErrorInvalidSymbol : e;
+    (void)hipGetLastError();   // the failure is reported by the entry point, not later calls
+  }
+}
+
+void* scratch_addr(int device, hipError_t* status) {
+  if (device < 0 || device >= kScratchDevices) {
+    *status = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  *status = g_scratch_status[device];
+  return g_scratch_addr[device];
+}
 
 int fail(jx_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -526,6 +568,42 @@ int step_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, double dt,
   return launch_step<T>(c, s, w, in, out, true, raw, false, hook);
 }
 
+// One step on the static scratch: the same plan and kernels as step_impl, with the stage
+// registers and partial sums carved out of `base` (this device's address of `scratch`).
+// Nothing there needs to start at zero: every stage writes its whole output register
+// before a later stage reads it, S is written from the step's input by the first row with
+// a delta, and each partial sum is written by its block before the reduction reads it.
+template <typename T>
+int step_noalloc_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, double dt,
+                      const T* raw, double dedt, unsigned char* base) {
+  const size_t bytes = (size_t)p->nx * p->ny * p->nvar * sizeof(T);
+  const size_t reg = (bytes + 255) & ~(size_t)255;
+  const RkTable tab = rk_table(p);
+  bool delta = false;
+  for (int k = 0; k < tab.n; ++k) delta = delta || tab.row[k].has_delta;
+  const size_t nreg = delta ? 3 : 2;
+  if (kNoAllocPartialBytes + nreg * reg > kNoAllocScratchBytes)
+    return fail(c, JX_EINVAL,
+                "jx_isothermal_step_noalloc: this state needs " + std::to_string(nreg) +
+                    " stage registers of " + std::to_string(bytes) + " bytes; the static "
+                    "scratch holds " + std::to_string(kNoAllocScratchBytes -
+                                                      kNoAllocPartialBytes) +
+                    " bytes of registers (use jx_isothermal_step for larger states)");
+  StepPlan<T> s = make_plan<T>(c, p, false);
+  const size_t nblocks = (size_t)s.grid.x * s.grid.y;
+  if (raw && kick_partials(nblocks) * sizeof(double) > kNoAllocPartialBytes)
+    return fail(c, JX_EINVAL, "jx_isothermal_step_noalloc: the kick needs partial sums for " +
+                                  std::to_string(nblocks) + " blocks; the static scratch "
+                                  "holds them for at most " +
+                                  std::to_string(kNoAllocPartialBytes / (7 * sizeof(double))));
+  unsigned char* regs = base + kNoAllocPartialBytes;
+  StepScratch<T> w{reinterpret_cast<T*>(regs), reinterpret_cast<T*>(regs + reg),
+                   delta ? reinterpret_cast<T*>(regs + 2 * reg) : nullptr,
+                   reinterpret_cast<double*>(base), false};
+  hipLaunchKernelGGL(jx::set_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, dt, dedt);
+  return launch_step<T>(c, s, w, in, out, true, raw, false, nullptr);
+}
+
 template <typename T>
 int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx_forcing* f,
                 double* t_out, double* dt_out, int* cyc_out, const SgsHook<T>* hook = nullptr,
@@ -793,6 +871,7 @@ int jx_ctx_create(jx_ctx** out, int device, void* stream) {
   if (e == hipSuccess) e = hipHostMalloc(&c->host_ctrl, sizeof(Ctrl), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&c->sums, 16 * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(&c->host_sums, 16 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) scratch_setup(device);
   if (e != hipSuccess) {
     std::fprintf(stderr, "jx_ctx_create: %s\n", hipGetErrorString(e));
     jx_ctx_destroy(c);
@@ -874,6 +953,29 @@ int jx_isothermal_step(jx_ctx* c, const jx_run_params* p, const void* U_in, void
                        double dt, const void* raw_force, double dedt) {
   return jx_isothermal_step_sgs(c, p, U_in, U_out, dt, raw_force, dedt, nullptr, nullptr,
                                 nullptr, nullptr);
+}
+
+int jx_isothermal_step_noalloc(jx_ctx* c, const jx_run_params* p, const void* U_in,
+                               void* U_out, double dt, const void* raw_force, double dedt) {
+  if (!c) return JX_EINVAL;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!U_in || !U_out || U_in == U_out)
+    return fail(c, JX_EINVAL, "U_in and U_out must be distinct device arrays");
+  hipError_t status = hipSuccess;
+  void* base = scratch_addr(c->device, &status);
+  if (!base)
+    return fail(c, JX_EHIP, std::string("jx_isothermal_step_noalloc: no static scratch on "
+                                        "device ") + std::to_string(c->device) + ": " +
+                                hipGetErrorString(status));
+  unsigned char* b = static_cast<unsigned char*>(base);
+  if (p->dtype == 0)
+    return step_noalloc_impl<double>(c, p, static_cast<const double*>(U_in),
+                                     static_cast<double*>(U_out), dt,
+                                     static_cast<const double*>(raw_force), dedt, b);
+  return step_noalloc_impl<float>(c, p, static_cast<const float*>(U_in),
+                                  static_cast<float*>(U_out), dt,
+                                  static_cast<const float*>(raw_force), dedt, b);
 }
 
 int jx_isothermal_step_sgs(jx_ctx* c, const jx_run_params* p, const void* U_in, void* U_out,
