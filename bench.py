@@ -61,6 +61,11 @@ def parse():
 
 
 def gpu_leg(jx, problems, n, dtype, steps, warmup, block_x, rows):
+    """`warmup` untimed steps, then `steps` timed steps of the device loop (wall clock
+    between two synchronisations: `value`), then the same `steps` again with every stage
+    launch timed by HIP events on the context stream (the roofline's launch duration).
+    The events are taken in their own pass because each one costs the launch stream a few
+    microseconds, which would otherwise be charged to the wall-clock figure."""
     import torch
     from jathenax_amd.device import context_for
     tdt = torch.float32 if dtype == "f32" else torch.float64
@@ -71,19 +76,23 @@ def gpu_leg(jx, problems, n, dtype, steps, warmup, block_x, rows):
     if warmup:
         jx.evolve_isothermal(U0, *args, warmup)
     torch.cuda.synchronize()
-    ctx.set_kernel_timing(True)
     t0 = time.perf_counter()
     U, t, dt, cyc = jx.evolve_isothermal(U0, *args, steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    assert cyc == steps, (cyc, steps)
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    jx.evolve_isothermal(U0, *args, steps)
+    torch.cuda.synchronize()
+    wall_events = time.perf_counter() - t0
     launches, kernel_ms = ctx.kernel_timing()
     ctx.set_kernel_timing(False)
-    assert cyc == steps, (cyc, steps)
     finite = bool(torch.isfinite(U).all())
-    del U
     from jathenax_amd.hydro import run_params
     used = ctx.tiles_for(run_params(U0, *args[:5]))
-    return dict(wall=wall, launches=launches, kernel_ms=kernel_ms, finite=finite, tiles=used)
+    return dict(wall=wall, wall_events=wall_events, launches=launches, kernel_ms=kernel_ms,
+                finite=finite, tiles=used, rho=U[..., 0].double())
 
 
 def ensemble_leg(n_inst):
@@ -303,6 +312,13 @@ def main():
             tiles[dt_name] = list(choice)
             del U0
         print(f"# bench: tuned tiles {tiles}", file=sys.stderr, flush=True)
+    # The FP64 leg (the reference point of the FP32/FP64 ratio) runs first: a card that has
+    # been idle needs some tens of milliseconds of load before its clocks settle, and the
+    # FP32 leg is the headline figure.
+    f64 = None
+    if not a.no_fp64 and a.dtype == "f32":
+        print(f"# bench: fp64 leg, steps={K} warmup={a.warmup}", file=sys.stderr, flush=True)
+        f64 = gpu_leg(jx, problems, n, "f64", K, a.warmup, a.block_x, a.rows)
     print(f"# bench: {n}x{n} {a.dtype} steps={K} warmup={a.warmup}", file=sys.stderr, flush=True)
     main_leg = gpu_leg(jx, problems, n, a.dtype, K, a.warmup, a.block_x, a.rows)
     zu = n * n * stages * K / main_leg["wall"]
@@ -324,16 +340,20 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(n, a.dtype, main_leg["tiles"]),
                      "kernel": "jx::stage_kernel (fused RK stage)",
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": avg_launch_s * 1e3, "launches_timed": main_leg["launches"]},
+                     "avg_launch_ms": avg_launch_s * 1e3, "launches_timed": main_leg["launches"],
+                     "ms_per_step_with_launch_events": main_leg["wall_events"] / K * 1e3},
         "finite": main_leg["finite"],
     }
-    if not a.no_fp64 and a.dtype == "f32":
-        print("# bench: fp64 leg", file=sys.stderr, flush=True)
-        f64 = gpu_leg(jx, problems, n, "f64", K, a.warmup, a.block_x, a.rows)
+    if f64 is not None:
         zu64 = n * n * stages * K / f64["wall"]
         out["fp64"] = {"value": zu64, "ms_per_step": f64["wall"] / K * 1e3,
                        "avg_launch_ms": f64["kernel_ms"] / max(1, f64["launches"]),
-                       "fp32_over_fp64": zu / zu64}
+                       "fp32_over_fp64": zu / zu64,
+                       "fp32_vs_fp64_l1_density_after_steps":
+                           float((main_leg["rho"] - f64["rho"]).abs().mean())}
+    del main_leg["rho"]
+    if f64 is not None:
+        del f64["rho"]
     if a.configs > 0:
         print("# bench: other workloads", file=sys.stderr, flush=True)
         out["configs"] = configs_leg(a.configs)

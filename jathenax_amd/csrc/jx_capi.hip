@@ -43,7 +43,7 @@ using jx::StageArgs;
 // Stage launchers, one per (precision, reconstruction) translation unit.
 #define JX_DECL(T, NAME)                                                                    \
   hipError_t NAME(int rs, int nv, bool visc, const jx::StageArgs<T>& a, dim3 grid, int bx, \
-                  hipStream_t st);                                                         \
+                  hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);                         \
   int NAME##_occupancy(int rs, int nv, bool visc, int bx, bool exact);
 JX_DECL(double, jx_stage_f64_dc)
 JX_DECL(double, jx_stage_f64_plm)
@@ -273,13 +273,13 @@ template <> struct StageLaunch<double> {
     }
   }
   static hipError_t go(int recon, int rs, int nv, bool visc, const StageArgs<double>& a,
-                       dim3 g, int bx, hipStream_t st) {
+                       dim3 g, int bx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     switch (recon) {
-      case jx::RECON_DC: return jx_stage_f64_dc(rs, nv, visc, a, g, bx, st);
-      case jx::RECON_PLM: return jx_stage_f64_plm(rs, nv, visc, a, g, bx, st);
-      case jx::RECON_PPM4: return jx_stage_f64_ppm4(rs, nv, visc, a, g, bx, st);
-      case jx::RECON_PPMX: return jx_stage_f64_ppmx(rs, nv, visc, a, g, bx, st);
-      default: return jx_stage_f64_wenoz(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_DC: return jx_stage_f64_dc(rs, nv, visc, a, g, bx, st, e0, e1);
+      case jx::RECON_PLM: return jx_stage_f64_plm(rs, nv, visc, a, g, bx, st, e0, e1);
+      case jx::RECON_PPM4: return jx_stage_f64_ppm4(rs, nv, visc, a, g, bx, st, e0, e1);
+      case jx::RECON_PPMX: return jx_stage_f64_ppmx(rs, nv, visc, a, g, bx, st, e0, e1);
+      default: return jx_stage_f64_wenoz(rs, nv, visc, a, g, bx, st, e0, e1);
     }
   }
 };
@@ -294,13 +294,13 @@ template <> struct StageLaunch<float> {
     }
   }
   static hipError_t go(int recon, int rs, int nv, bool visc, const StageArgs<float>& a,
-                       dim3 g, int bx, hipStream_t st) {
+                       dim3 g, int bx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     switch (recon) {
-      case jx::RECON_DC: return jx_stage_f32_dc(rs, nv, visc, a, g, bx, st);
-      case jx::RECON_PLM: return jx_stage_f32_plm(rs, nv, visc, a, g, bx, st);
-      case jx::RECON_PPM4: return jx_stage_f32_ppm4(rs, nv, visc, a, g, bx, st);
-      case jx::RECON_PPMX: return jx_stage_f32_ppmx(rs, nv, visc, a, g, bx, st);
-      default: return jx_stage_f32_wenoz(rs, nv, visc, a, g, bx, st);
+      case jx::RECON_DC: return jx_stage_f32_dc(rs, nv, visc, a, g, bx, st, e0, e1);
+      case jx::RECON_PLM: return jx_stage_f32_plm(rs, nv, visc, a, g, bx, st, e0, e1);
+      case jx::RECON_PPM4: return jx_stage_f32_ppm4(rs, nv, visc, a, g, bx, st, e0, e1);
+      case jx::RECON_PPMX: return jx_stage_f32_ppmx(rs, nv, visc, a, g, bx, st, e0, e1);
+      default: return jx_stage_f32_wenoz(rs, nv, visc, a, g, bx, st, e0, e1);
     }
   }
 };
@@ -422,7 +422,8 @@ int ctx_scratch(jx_ctx* c, size_t state_bytes, size_t nblocks, bool kick, StepSc
 }
 
 // Launch the RK stages of one step.  `first`: the saved register is the caller's input
-// (floored on read).  The result lands in `A`.  `raw`: kick the final stage.
+// (floored on read).  The result lands in `A`.  `raw`: kick the final stage.  `want_dtkey`:
+// the step's last kernel reduces the next CFL limit.
 template <typename T>
 int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const T* in, T* A,
                 bool first, const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr) {
@@ -491,12 +492,11 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
         c->events.push_back(pr);
       }
       ev = &c->events[c->events_used++];
-      JX_HIP(c, hipEventRecord(ev->first, c->stream));
     }
     hipError_t e = StageLaunch<T>::go(p->reconstruct, p->rsolver, p->nvar, s.visc, a, s.grid,
-                                      s.bx, c->stream);
+                                      s.bx, c->stream, ev ? ev->first : nullptr,
+                                      ev ? ev->second : nullptr);
     if (e != hipSuccess) return hip_fail(c, e, "stage kernel launch");
-    if (ev) JX_HIP(c, hipEventRecord(ev->second, c->stream));
     if (last && a.u_out != A)
       JX_HIP(c, hipMemcpyAsync(A, a.u_out, (size_t)p->nx * p->ny * p->nvar * sizeof(T),
                                hipMemcpyDeviceToDevice, c->stream));
@@ -673,23 +673,37 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     }
   }
   const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
-  // Initial timestep from the caller's state (run.py:42-46).
+  // Initial timestep from the caller's state (run.py:42-46).  Each step opens with the
+  // one-thread begin_step kernel (commit the last step, test the loop condition, pick dt).
   hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
                      c->stream, timestep_args<T>(c, p, in, 0, c->ctrl));
-  auto one_step = [&](bool first) -> int {
-    hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
+  const jx::TimestepArgs<T> ts = timestep_args<T>(c, p, out, 0, c->ctrl);
+  // `open`: launch the step's begin_step (false for the first step after a chunk boundary,
+  // which the boundary's begin_step has opened already).
+  auto one_step = [&](bool first, bool open) -> int {
+    if (open)
+      hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
     if (forced) launch_render<T>(c, ra);
     int r = launch_step<T>(c, s, w, in, out, first, raw, !smag, hook);
     if (r) return r;
     if (smag)
       hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
-                         c->stream, timestep_args<T>(c, p, out, 0, c->ctrl));
+                         c->stream, ts);
     return JX_OK;
   };
-  rc = one_step(true);
-  if (rc) return rc;
-  long long chunk = 1;
+  // The first chunk goes out before dt is read back: up to 32 steps (kernels launched past
+  // the end of the loop exit at once).  With a stress callback every step is its own chunk:
+  // the callback is host code, and steps past the end must not call it.
+  long long chunk = hook ? 1 : std::min<long long>(32, (long long)p->nlim - s0.cycle);
+  bool first = true, opened = false;
   for (;;) {
+    for (long long k = 0; k < chunk; ++k) {
+      rc = one_step(first, !(k == 0 && opened));
+      if (rc) return rc;
+      first = false;
+    }
+    opened = true;
+    // Commit the last launched step (and open the next) before reading the loop state.
     hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
     JX_HIP(c, hipMemcpyAsync(c->host_ctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost,
                              c->stream));
@@ -702,27 +716,12 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
       if (end) *end = jx_loop_state{h.time, h.prev_dt, h.cycle, h.update};
       return JX_OK;
     }
-    // The begin_step above already opened the next step: launch it, then a chunk more.
-    // Remaining steps are estimated from the time left; dt can shrink, so loop.
+    // The next step is open: launch it and a chunk more.  Remaining steps are estimated
+    // from the time left; dt can shrink, so loop.
     double est = (p->tlim - h.time) / h.dt;
     long long want = std::isfinite(est) ? (long long)std::ceil(est) : 1;
     want = std::min<long long>(want, (long long)p->nlim - h.cycle);
-    // With a stress callback every step is its own chunk: the callback is host code, and
-    // steps past the end must not call it.
     chunk = hook ? 1 : std::max<long long>(1, std::min<long long>(want, 512));
-    {
-      if (forced) launch_render<T>(c, ra);
-      int r = launch_step<T>(c, s, w, in, out, false, raw, !smag, hook);
-      if (r) return r;
-      if (smag)
-        hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny),
-                           dim3(jx::kAuxBlock), 0, c->stream,
-                           timestep_args<T>(c, p, out, 0, c->ctrl));
-    }
-    for (long long k = 1; k < chunk; ++k) {
-      rc = one_step(false);
-      if (rc) return rc;
-    }
   }
 }
 

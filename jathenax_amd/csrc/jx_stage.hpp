@@ -15,10 +15,12 @@
 // floor (hydro.py:54-57) and, in the epilogue, either the CFL min for the next timestep
 // (hydro.py:66-68) or the first moments of the forcing normalisation (forcing.py:141).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
+#include "jx_loop.hpp"
 #include "jx_ops.hpp"
 
 namespace jx {
@@ -533,7 +535,7 @@ stage_kernel(StageArgs<T> args) {
           // RK4 saved-register accumulation (hydro.py:42-43)
 #pragma unroll
           for (int v = 0; v < NV; ++v) us[v] = us[v] + T(a.delta) * u[S3][v];
-          row_store<T, NV>(row_rsrc(a.s_out + (size_t)j * row_len, row_bytes), lane_out, us);
+          row_store<T, NV, kNonTemporal>(row_rsrc(a.s_out + (size_t)j * row_len, row_bytes), lane_out, us);
         }
         if constexpr (Fast<T>::value) {
           T dF[NV], dG[NV];
@@ -588,7 +590,7 @@ stage_kernel(StageArgs<T> args) {
             cfl.add(inv * un[1], inv * un[2]);
           }
         }
-        row_store<T, NV>(row_rsrc(out_row, row_bytes), lane_out, un);
+        row_store<T, NV, kNonTemporal>(row_rsrc(out_row, row_bytes), lane_out, un);
       }
     }
     // gn is set from iteration 5 on and gp is first read at iteration 6, so the copy can
@@ -606,13 +608,47 @@ stage_kernel(StageArgs<T> args) {
   body(IC<3>{}, Warm{}, 3);
   body(IC<4 % R>{}, Warm{}, 4);
   body(IC<5 % R>{}, Warm{}, 5);
-  for (int i0 = 6; i0 < nit; i0 += R) {
-    body(IC<6 % R>{}, Steady{}, i0);
-    if (i0 + 1 < nit) body(IC<7 % R>{}, Steady{}, i0 + 1);
-    if (i0 + 2 < nit) body(IC<8 % R>{}, Steady{}, i0 + 2);
-    if (i0 + 3 < nit) body(IC<9 % R>{}, Steady{}, i0 + 3);
+  auto round = [&](auto k0, int i) {
+    constexpr int K = decltype(k0)::value;
+    body(IC<(6 + K) % R>{}, Steady{}, i + K);
+  };
+  int i0 = 6;
+  constexpr bool WHOLE = Fast<T>::value && NV == 3 && !VISC && RECON != RECON_PPMX &&
+                        RECON != RECON_WENOZ;
+  if constexpr (WHOLE) {
+    // FP32 hydro without viscosity (DC, PLM, PPM4): the steady loop runs whole rounds of two
+    // rings' worth of rows (8 or 10), with no exit between them.  An exit after a row makes the ring and prefetch registers live-out
+    // there, and the copies that merge them wait for every load in flight; the round's back
+    // edge is the only such point, once per 2R rows.  The last (nit - 6) mod 2R rows follow.
+    constexpr int RR = 2 * R;
+    for (; i0 + RR <= nit; i0 += RR) {
+      round(IC<0>{}, i0); round(IC<1>{}, i0); round(IC<2>{}, i0); round(IC<3>{}, i0);
+      round(IC<4>{}, i0); round(IC<5>{}, i0); round(IC<6>{}, i0); round(IC<7>{}, i0);
+      if constexpr (R == 5) { round(IC<8>{}, i0); round(IC<9>{}, i0); }
+    }
+    if (i0 + 0 < nit) round(IC<0>{}, i0);
+    if (i0 + 1 < nit) round(IC<1>{}, i0);
+    if (i0 + 2 < nit) round(IC<2>{}, i0);
+    if (i0 + 3 < nit) round(IC<3>{}, i0);
+    if (i0 + 4 < nit) round(IC<4>{}, i0);
+    if (i0 + 5 < nit) round(IC<5>{}, i0);
+    if (i0 + 6 < nit) round(IC<6>{}, i0);
     if constexpr (R == 5) {
-      if (i0 + 4 < nit) body(IC<10 % R>{}, Steady{}, i0 + 4);
+      if (i0 + 7 < nit) round(IC<7>{}, i0);
+      if (i0 + 8 < nit) round(IC<8>{}, i0);
+    }
+  } else {
+    // Rounds of one ring with a test before each row.  For FP64 (twice the registers per
+    // value), scalars, the viscous kernels and the PPMX / WENO-Z stencils the whole-round
+    // form makes the compiler keep part of the state in scratch memory.
+    for (; i0 < nit; i0 += R) {
+      round(IC<0>{}, i0);
+      if (i0 + 1 < nit) round(IC<1>{}, i0);
+      if (i0 + 2 < nit) round(IC<2>{}, i0);
+      if (i0 + 3 < nit) round(IC<3>{}, i0);
+      if constexpr (R == 5) {
+        if (i0 + 4 < nit) round(IC<4>{}, i0);
+      }
     }
   }
 
@@ -624,20 +660,32 @@ stage_kernel(StageArgs<T> args) {
   }
 }
 
+// One launch.  With events (timing), the start and stop timestamps are taken by the
+// dispatch itself (hipExtLaunchKernel), so timing adds no packets between kernels.
+template <typename T, typename K>
+hipError_t launch_one(K kernel, dim3 grid, int bx, size_t smem, hipStream_t st,
+                      const StageArgs<T>& a, hipEvent_t ev0, hipEvent_t ev1) {
+  if (ev0) {
+    void* args[] = {const_cast<StageArgs<T>*>(&a)};
+    return hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), grid, dim3(bx), args, smem,
+                              st, ev0, ev1, 0);
+  }
+  hipLaunchKernelGGL(kernel, grid, dim3(bx), smem, st, a);
+  return hipGetLastError();
+}
+
 // Host-side launch for one (precision, reconstruction) pair; the Riemann solver, the
 // number of variables and the viscous switch are dispatched here.
 template <typename T, int RECON, bool EXACT>
 hipError_t launch_stage_e(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 grid, int bx,
-                          hipStream_t st) {
+                          hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
 #define JX_LAUNCH(RS_, NV_, V_)                                                             \
   do {                                                                                     \
     if (a.stage1)                                                                          \
-      hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_, EXACT, true>), grid,        \
-                         dim3(bx), stage_smem_bytes<T>(NV_, V_, bx), st, a);               \
-    else                                                                                   \
-      hipLaunchKernelGGL((stage_kernel<T, RECON, RS_, NV_, V_, EXACT, false>), grid,       \
-                         dim3(bx), stage_smem_bytes<T>(NV_, V_, bx), st, a);               \
-    return hipGetLastError();                                                              \
+      return launch_one<T>(stage_kernel<T, RECON, RS_, NV_, V_, EXACT, true>, grid, bx,     \
+                           stage_smem_bytes<T>(NV_, V_, bx), st, a, ev0, ev1);             \
+    return launch_one<T>(stage_kernel<T, RECON, RS_, NV_, V_, EXACT, false>, grid, bx,      \
+                         stage_smem_bytes<T>(NV_, V_, bx), st, a, ev0, ev1);               \
   } while (0)
 #define JX_NV(RS_)                                                                          \
   do {                                                                                     \
@@ -693,12 +741,12 @@ int stage_occupancy(int rs, int nv, bool visc, int bx, bool exact) {
 // of two (a.exact), where the product equals the IEEE quotient.
 template <typename T, int RECON>
 hipError_t launch_stage(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 grid, int bx,
-                        hipStream_t st) {
+                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   if constexpr (Fast<T>::value) {
-    return launch_stage_e<T, RECON, true>(rs, nv, visc, a, grid, bx, st);
+    return launch_stage_e<T, RECON, true>(rs, nv, visc, a, grid, bx, st, ev0, ev1);
   } else {
-    if (a.exact) return launch_stage_e<T, RECON, true>(rs, nv, visc, a, grid, bx, st);
-    return launch_stage_e<T, RECON, false>(rs, nv, visc, a, grid, bx, st);
+    if (a.exact) return launch_stage_e<T, RECON, true>(rs, nv, visc, a, grid, bx, st, ev0, ev1);
+    return launch_stage_e<T, RECON, false>(rs, nv, visc, a, grid, bx, st, ev0, ev1);
   }
 }
 
@@ -708,8 +756,8 @@ hipError_t launch_stage(int rs, int nv, bool visc, const StageArgs<T>& a, dim3 g
 // occupancy query.
 #define JX_STAGE_LAUNCHER(T, RECON, NAME)                                                   \
   hipError_t NAME(int rs, int nv, bool visc, const jx::StageArgs<T>& a, dim3 grid, int bx,  \
-                  hipStream_t st) {                                                        \
-    return jx::launch_stage<T, RECON>(rs, nv, visc, a, grid, bx, st);                      \
+                  hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {                        \
+    return jx::launch_stage<T, RECON>(rs, nv, visc, a, grid, bx, st, ev0, ev1);            \
   }                                                                                        \
   int NAME##_occupancy(int rs, int nv, bool visc, int bx, bool exact) {                    \
     return jx::stage_occupancy<T, RECON>(rs, nv, visc, bx, exact);                         \
