@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--dtype", choices=("f32", "f64"), default="f32")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=2048, help="CPU baseline sample grid size")
+    ap.add_argument("--cpu-n", type=int, default=4096, help="CPU baseline sample grid size")
     ap.add_argument("--block-x", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--stream-ceiling", action="store_true",
@@ -52,9 +52,9 @@ def parse():
                     help="use the stored tiles (the default; kept for older command lines)")
     ap.add_argument("--ensemble", type=int, default=256,
                     help="instances in the batched-ensemble leg (0 = skip)")
-    ap.add_argument("--serve", type=int, default=16,
+    ap.add_argument("--serve", type=int, default=100,
                     help="requests in the serving leg (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, 1/s")
+    ap.add_argument("--serve-rate", type=float, default=3.0, help="Poisson arrival rate, 1/s")
     ap.add_argument("--configs", type=int, default=20,
                     help="steps per workload in the KH 1024^2 / turbulence 4096^2 leg (0 = skip)")
     return ap.parse_args()
@@ -208,7 +208,8 @@ def serving_leg(n_req, rate, sample=3):
     work = sum(workloads.zone_stages(r.spec, r.info.get("cycles", 0)) for r in out)
     rep = Server.report(out, wall, work)
     rep["workload"] = (f"{n_req} requests, Poisson rate {rate}/s, square grids log-uniform "
-                       "128..2048, tlim U[0.1, 2], KH / driven turbulence, PPM4+Roe RK3, FP32")
+                       "128..2048, tlim U[0.1, 2], KH / driven turbulence, PPM4+Roe RK3, FP32; "
+                       "latency = finish - arrival = queue (admitted - arrival) + service")
     got = {r.rid: r for r in out if r.rid in keep and r.status == "done"}
     for r in out:
         if r.rid not in got:
@@ -230,14 +231,25 @@ def serving_leg(n_req, rate, sample=3):
     return rep
 
 
-def cpu_leg(problems, n):
+def cpu_leg(problems, n, kh_steps=5):
+    """The NumPy float64 oracle on one host core: one RK3 step of the headline workload at
+    n x n, and `kh_steps` steps of Kelvin-Helmholtz 1024^2 with a tracer (nvar 4)."""
     sys.path.insert(0, str(ROOT))
     from oracle import jathenax_np as ox
     U = problems.smooth_wave(n, n)
     t0 = time.perf_counter()
     ox.isothermal_step(U, 1e-4, 1.0 / n, 1.0 / n, 1.0, 1e-10, "rk3")
     sec = time.perf_counter() - t0
-    return n * n * 3 / sec, sec
+    del U
+    kh = None
+    if kh_steps:
+        U = problems.kelvin_helmholtz_tanh(1024, 1024)
+        t0 = time.perf_counter()
+        ox.evolve_isothermal(U, 1 / 1024, 1 / 1024, 1.0, 1e-10, "rk3", 0.4, 1e9, kh_steps)
+        ksec = time.perf_counter() - t0
+        kh = {"value": 1024 * 1024 * 3 * kh_steps / ksec, "unit": "zone-updates/s",
+              "sample": f"{kh_steps} RK3 steps of KH 1024^2 with tracer, PPM4+Roe ({ksec:.1f} s)"}
+    return n * n * 3 / sec, sec, kh
 
 
 def stream_leg(nelem, reps=50):
@@ -368,12 +380,13 @@ def main():
         out["stream_ceiling"] = stream_leg(n * n * nvar)
     if not a.no_cpu_baseline:
         print("# bench: cpu baseline", file=sys.stderr, flush=True)
-        rate, sec = cpu_leg(problems, a.cpu_n)
+        rate, sec, kh = cpu_leg(problems, a.cpu_n)
         out["cpu_baseline"] = {
             "value": rate, "unit": "zone-updates/s", "cores": 1, "kind": "port",
             "sample": f"NumPy float64 oracle, one RK3 step of the same workload on a "
                       f"{a.cpu_n}x{a.cpu_n} grid ({sec:.1f} s; host has {os.cpu_count()} cores, "
-                      f"NumPy elementwise ops use one)"}
+                      f"NumPy elementwise ops use one)",
+            "kh_1024": kh}
     print(json.dumps(out), flush=True)
 
 

@@ -261,8 +261,19 @@ class Server:
 
     @staticmethod
     def report(finished, wall, zone_stages):
-        lat = np.asarray([r.latency for r in finished if r.status == "done"])
+        """Completion latency (finish - arrival) and its two parts, the time queued before
+        admission (admitted - arrival) and the time in service (finish - admitted), as
+        p50 / p99 over the completed requests."""
+        done = [r for r in finished if r.status == "done"]
+        lat = np.asarray([r.latency for r in done])
+        queue = np.asarray([r.admitted - r.arrival for r in done])
+        service = np.asarray([r.finished - r.admitted for r in done])
+
+        def pct(x, q):
+            return float(np.percentile(x, q)) if x.size else None
+
         return dict(completed=int(lat.size), errors=sum(r.status == "error" for r in finished),
-                    p50_s=float(np.percentile(lat, 50)) if lat.size else None,
-                    p99_s=float(np.percentile(lat, 99)) if lat.size else None,
+                    p50_s=pct(lat, 50), p99_s=pct(lat, 99),
+                    queue_p50_s=pct(queue, 50), queue_p99_s=pct(queue, 99),
+                    service_p50_s=pct(service, 50), service_p99_s=pct(service, 99),
                     wall_s=wall, zone_updates_per_s=zone_stages / wall if wall > 0 else None)

@@ -12,8 +12,7 @@ so a 4096^2 state is reduced in place without a host copy of the field.
 
     python -m jathenax_amd.spectra snapshot.npz [--output spectra.csv]
 
-writes the reference's CSV (mode,kinetic_energy,enstrophy) and, when matplotlib is present,
-its two-panel figure.
+writes the reference's CSV (mode,kinetic_energy,enstrophy); plotting is left to the reader.
 """
 
 import argparse
@@ -83,32 +82,14 @@ def write_csv(path, mode, energy, enstrophy):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="Shell kinetic energy and enstrophy of a snapshot")
     ap.add_argument("snapshot", type=Path)
-    ap.add_argument("--output", type=Path, help="Figure or .csv path")
+    ap.add_argument("--output", type=Path, help=".csv path (default: <snapshot>_spectra.csv)")
     a = ap.parse_args(argv)
-    out = a.output or a.snapshot.with_name(a.snapshot.stem + "_spectra.png")
+    out = (a.output or a.snapshot.with_name(a.snapshot.stem + "_spectra")).with_suffix(".csv")
     with np.load(a.snapshot) as snap:
         U = snap["conserved"]
-        time = float(snap["time"])
-    ny, nx = U.shape[:2]
     mode, energy, enstrophy = shell_spectra(torch.as_tensor(U))
     out.parent.mkdir(parents=True, exist_ok=True)
-    write_csv(out.with_suffix(".csv"), mode, energy, enstrophy)
-    if out.suffix != ".csv":
-        try:
-            import matplotlib
-            matplotlib.use("Agg")
-            import matplotlib.pyplot as plt
-        except ImportError:
-            print(out.with_suffix(".csv"))
-            return
-        complete = slice(1, min(nx, ny) // 2)
-        fig, axes = plt.subplots(1, 2, figsize=(8.5, 3.6), layout="constrained")
-        for axis, values, label in zip(axes, (energy, enstrophy), ("Kinetic energy", "Enstrophy")):
-            axis.loglog(mode[complete], values[complete])
-            axis.set(xlabel="Box mode", ylabel=label)
-            axis.grid(alpha=0.2)
-        fig.suptitle(f"t = {time:g}")
-        fig.savefig(out, dpi=160)
+    write_csv(out, mode, energy, enstrophy)
     print(out)
 
 
