@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=3.0, help="Poisson arrival rate, 1/s")
     ap.add_argument("--configs", type=int, default=20,
                     help="steps per workload in the KH 1024^2 / turbulence 4096^2 leg (0 = skip)")
+    ap.add_argument("--config-set", choices=("all", "kh", "turbulence"), default="all",
+                    help="which workloads the configs leg runs")
     return ap.parse_args()
 
 
@@ -152,7 +154,7 @@ def ensemble_leg(n_inst):
                         "rsolver llf/hlle/roe, rk2/rk3, FP32, one batched engine"}
 
 
-def configs_leg(steps):
+def configs_leg(steps, which="all"):
     """Throughput of the other single-GPU workloads, device-resident loop, `steps` steps each
     after one warm-up step: Kelvin-Helmholtz 1024^2 with tracer (PPM4+Roe, RK3) in FP32 and
     FP64, and driven turbulence 4096^2 FP32 (WENO-Z+HLLE, RK2, CFL 0.3, forcing refreshed
@@ -161,7 +163,8 @@ def configs_leg(steps):
     from jathenax_amd import problems, run, workloads
     out = {}
     U = problems.kelvin_helmholtz_tanh(1024, 1024)
-    for name, tdt in (("kh_1024_f32", torch.float32), ("kh_1024_f64", torch.float64)):
+    kh = (("kh_1024_f32", torch.float32), ("kh_1024_f64", torch.float64))
+    for name, tdt in kh if which in ("all", "kh") else ():
         U0 = torch.tensor(U, dtype=tdt, device="cuda")
         args = (1 / 1024, 1 / 1024, 1.0, 1e-10, "rk3", 0.4, 1e9)
         run.evolve_isothermal(U0, *args, 1)
@@ -172,6 +175,9 @@ def configs_leg(steps):
         w = time.perf_counter() - t0
         out[name] = {"zone_updates_per_s": 1024 * 1024 * 3 * steps / w,
                      "ms_per_step": w / steps * 1e3}
+    out["steps"] = steps
+    if which == "kh":
+        return out
     s = workloads.turbulence_spec(4096, 4096, 0.01, 1618, 5e-5, "rk2", "wenoz", "hlle", cfl=0.3,
                                   dtype=np.float32)
     U0 = torch.tensor(s["U"], dtype=torch.float32, device="cuda")
@@ -184,7 +190,6 @@ def configs_leg(steps):
     w = time.perf_counter() - t0
     out["turbulence_4096_f32"] = {"zone_updates_per_s": 4096 * 4096 * 2 * steps / w,
                                   "ms_per_step": w / steps * 1e3}
-    out["steps"] = steps
     return out
 
 
@@ -376,7 +381,7 @@ def main():
         del f64["U"], U_f64
     if a.configs > 0:
         print("# bench: other workloads", file=sys.stderr, flush=True)
-        out["configs"] = configs_leg(a.configs)
+        out["configs"] = configs_leg(a.configs, a.config_set)
     if a.ensemble > 0:
         print(f"# bench: ensemble leg ({a.ensemble} instances)", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_leg(a.ensemble)

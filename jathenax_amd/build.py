@@ -58,9 +58,17 @@ def compile_commands():
     return cmds
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, only=None):
+    """Compile every translation unit (or, with `only`, just the named source stems, reusing
+    the other objects of an earlier build) and link the library."""
     OBJ.mkdir(exist_ok=True)
     cmds = compile_commands()
+    if only:
+        missing = [obj for src, obj, _ in cmds if src.stem not in only and not obj.exists()]
+        if missing:
+            raise RuntimeError(f"no earlier object for {missing[0].name}: run a full build")
+        keep = [(src, obj, None) for src, obj, _ in cmds if src.stem not in only]
+        cmds = [c for c in cmds if c[0].stem in only]
 
     def run(item):
         src, obj, cmd = item
@@ -73,6 +81,8 @@ def build(jobs=None, verbose=False):
 
     with ThreadPoolExecutor(_jobs(jobs)) as pool:
         objs = list(pool.map(run, cmds))
+    if only:
+        objs = sorted(objs + [obj for _, obj, _ in keep])
     tmp = LIB.with_suffix(".so.tmp")
     link = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
             *map(str, objs)]
@@ -89,6 +99,8 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", "--jobs", type=int)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--only", nargs="+", metavar="STEM",
+                    help="recompile only these sources (e.g. jx_capi), keep the other objects")
     args = ap.parse_args()
-    print(build(args.jobs, args.verbose))
+    print(build(args.jobs, args.verbose, args.only))
     sys.exit(0)

@@ -40,6 +40,11 @@ template <typename T> struct TimestepArgs {
   const Ctrl* ctrl;               // nullptr: always run
   unsigned long long* dtkey;      // min slot
   unsigned long long* nukey;      // max slot
+  // timestep_kernel<T, true> only: q holds the post-stage state of a forced step, raw its
+  // raw force, and the kicked state goes to out (another buffer than q).
+  const T* raw;
+  T* out;
+  int waves;                      // target wave count of the tiling (ts_tiling)
 };
 
 __device__ __forceinline__ unsigned long long max_key_of(double x) {
@@ -51,18 +56,21 @@ __device__ __forceinline__ unsigned long long max_key_of(double x) {
 // columns (lanes 1..62; lanes 0 and 63 carry the neighbouring columns) and a band of rows,
 // with about 8192 waves in all (32 per CU) so that other waves cover each row's load.
 constexpr int kTsCols = 62;
+constexpr int kTsAhead = 4;   // rows in flight per lane in the Smagorinsky march
+constexpr int kTsWaves = 8192;       // waves of a timestep reduction (32 per CU)
+constexpr int kTsKickWaves = 4096;   // with the kick: one resident round at 4 waves per SIMD
 struct TsTiling {
   int nstrips, nbands, rows;
 };
-__host__ __device__ inline TsTiling ts_tiling(int nx, int ny) {
+__host__ __device__ inline TsTiling ts_tiling(int nx, int ny, int waves = 8192) {
   const int nstrips = (nx + kTsCols - 1) / kTsCols;
-  int nbands = (8192 + nstrips - 1) / nstrips;
+  int nbands = (waves + nstrips - 1) / nstrips;
   nbands = nbands > ny ? ny : (nbands < 1 ? 1 : nbands);
   const int rows = (ny + nbands - 1) / nbands;
   return {nstrips, (ny + rows - 1) / rows, rows};
 }
-inline dim3 ts_grid(int nx, int ny) {
-  const TsTiling t = ts_tiling(nx, ny);
+inline dim3 ts_grid(int nx, int ny, int waves = 8192) {
+  const TsTiling t = ts_tiling(nx, ny, waves);
   const int waves = t.nstrips * t.nbands;
   return dim3((unsigned)((waves + kAuxBlock / 64 - 1) / (kAuxBlock / 64)));
 }
@@ -72,16 +80,33 @@ inline dim3 ts_grid(int nx, int ny) {
 // the adjacent lanes (one shuffle per value and row), so the Smagorinsky face stencils cost
 // no further loads. The face expressions are those of fluxes.py:227-245 in the reference
 // order, so the limits match the cell-by-cell form bit for bit.
-template <typename T>
+//
+// KICK: the kick that ends a forced step (forcing.py:149, hydro.py:127-132; mu, scale and
+// dt from the control block) is applied to each row as it enters, before its primitives are
+// recovered, and the lanes that own a cell write its kicked state to `out`.  Halo rows and
+// columns are kicked by every wave that reads them, with the owner's arithmetic, so the
+// limits are those of the kicked state; `out` must differ from `q`, which neighbouring waves
+// are still reading.  The expressions are kick_apply_kernel's, so the state matches the
+// separate kick bit for bit.
+template <typename T, bool KICK = false>
 __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) {
   if (a.ctrl && a.ctrl->done) return;
   const int nx = a.nx, ny = a.ny, nv = a.nv;
-  const TsTiling tl = ts_tiling(nx, ny);
+  const TsTiling tl = ts_tiling(nx, ny, a.waves);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(
       (int)(blockIdx.x * (kAuxBlock / 64) + (threadIdx.x >> 6)));
   CflMin<T> cfl(a.dx, a.dy, a.cs);
   unsigned long long kmax = 0ull;
+  T mu0 = T(0), mu1 = T(0), kdt = T(0), scale = T(0);
+  bool off = true;
+  if constexpr (KICK) {
+    mu0 = T(a.ctrl->mu[0]);
+    mu1 = T(a.ctrl->mu[1]);
+    kdt = T(a.ctrl->dt);
+    off = a.ctrl->power == 0.0;
+    scale = T(a.ctrl->scale);
+  }
   if (wave < tl.nstrips * tl.nbands) {
     const int strip = wave % tl.nstrips, band = wave / tl.nstrips;
     const int x0 = strip * kTsCols;   // first output column
@@ -89,58 +114,116 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
     if (xc < 0) xc += nx;
     const bool out = lane >= 1 && lane <= kTsCols && x0 + lane - 1 < nx;
     const int j0 = band * tl.rows, j1 = min(j0 + tl.rows, ny);
-    // Raw values of a row (periodic), and the primitives recovered from them.
-    auto fetch = [&](int j, T (&q)[3]) {
+    // Raw values of a row (periodic): the state, and with KICK the raw force and the
+    // passive variables (nvar <= 5) that the kicked copy carries.
+    struct Row {
+      T q[3];
+      T f[KICK ? 2 : 1];
+      T x[KICK ? 2 : 1];
+    };
+    auto fetch = [&](int j, Row& w) {
       if (j < 0 || j >= ny) {
         j %= ny;
         if (j < 0) j += ny;
       }
-      const T* src = a.q + ((size_t)j * nx + xc) * nv;
-      q[0] = src[0];
-      q[1] = src[1];
-      q[2] = src[2];
+      const size_t cell = (size_t)j * nx + xc;
+      const T* src = a.q + cell * nv;
+      if constexpr (KICK) {
+        // one vector load per lane and array (rows as buffer resources)
+        const int row_bytes = nx * nv * (int)sizeof(T);
+        const unsigned lane_off = (unsigned)(xc * nv * (int)sizeof(T));
+        if (nv == 3) {
+          row_load<T, 3>(row_rsrc(a.q + (size_t)j * nx * 3, row_bytes), lane_off, w.q);
+          w.x[0] = w.x[1] = T(0);
+        } else {
+          w.q[0] = src[0];
+          w.q[1] = src[1];
+          w.q[2] = src[2];
+          w.x[0] = src[3];
+          w.x[1] = nv > 4 ? src[4] : T(0);
+        }
+        row_load<T, 2>(row_rsrc(a.raw + (size_t)j * nx * 2, nx * 2 * (int)sizeof(T)),
+                       (unsigned)(xc * 2 * (int)sizeof(T)), w.f);
+      } else {
+        w.q[0] = src[0];
+        w.q[1] = src[1];
+        w.q[2] = src[2];
+      }
     };
-    auto prim = [&](const T (&q)[3], T& u, T& v) {
+    // Row j enters: kick it (KICK; the owned cells are written), then recover.
+    auto prim = [&](int j, Row& w, T& u, T& v) {
+      if constexpr (KICK) {
+        const T f0 = w.f[0] - mu0, f1 = w.f[1] - mu1;
+        const T acc0 = off ? T(0) : scale * f0;
+        const T acc1 = off ? T(0) : scale * f1;
+        const T rho = w.q[0];
+        const T m0 = w.q[1] + rho * (acc0 * kdt);
+        const T m1 = w.q[2] + rho * (acc1 * kdt);
+        w.q[0] = floor_to(rho, a.dfloor);
+        w.q[1] = m0;
+        w.q[2] = m1;
+        if (out && j >= j0 && j < j1) {
+          if (nv == 3) {
+            row_store<T, 3>(row_rsrc(a.out + (size_t)j * nx * 3,
+                                                   nx * 3 * (int)sizeof(T)),
+                                          (unsigned)(xc * 3 * (int)sizeof(T)), w.q);
+          } else {
+            T* dst = a.out + ((size_t)j * nx + xc) * nv;
+            dst[0] = w.q[0];
+            dst[1] = m0;
+            dst[2] = m1;
+            dst[3] = w.x[0];
+            if (nv > 4) dst[4] = w.x[1];
+          }
+        }
+      }
       T r;
-      load_prim(q, nv, a.primitive != 0, a.dfloor, r, u, v);
+      load_prim(w.q, nv, a.primitive != 0, a.dfloor, r, u, v);
     };
     // Rows are fetched two ahead of their use, so their latency overlaps the work of the
     // rows in between.
-    T q1[3], q2[3];
+    Row q1, q2;
     fetch(j0, q1);
     fetch(j0 + 1, q2);
     if (!a.smag_on) {
       for (int j = j0; j < j1; ++j) {
         T u, v;
-        prim(q1, u, v);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+        prim(j, q1, u, v);
+        q1 = q2;
         fetch(j + 2, q2);
         if (out) cfl.add(u, v);
       }
     } else {
       // Rows s = j-1, c = j of this column (o), the one to the left (l) and to the right
-      // (r): velocities vx (u) and vy (v).
+      // (r): velocities vx (u) and vy (v).  Rows j0 .. j0+kTsAhead-1 are fetched up front
+      // into a ring (row r in slot (r - j0) mod kTsAhead); each row's slot is refilled with
+      // the row kTsAhead further up as soon as the row has entered.  The loop runs whole
+      // rounds of kTsAhead rows, so every slot index is a constant and no register copy has
+      // to wait for a load in flight.
+      constexpr int D = kTsAhead;
+      Row ring[D];
+      ring[0] = q1;
+      ring[1] = q2;
+#pragma unroll
+      for (int k = 2; k < D; ++k) fetch(j0 + k, ring[k]);
       T ou_s, ov_s, ou_c, ov_c;
       {
-        T q0[3];
+        Row q0;
         fetch(j0 - 1, q0);
-        prim(q0, ou_s, ov_s);
+        prim(j0 - 1, q0, ou_s, ov_s);
       }
-      prim(q1, ou_c, ov_c);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q1[k] = q2[k];
-      fetch(j0 + 2, q2);
+      prim(j0, ring[0], ou_c, ov_c);
+      fetch(j0 + D, ring[0]);
       T lu_s = __shfl_up(ou_s, 1), lv_s = __shfl_up(ov_s, 1);
       T lu_c = __shfl_up(ou_c, 1), lv_c = __shfl_up(ov_c, 1);
       T ru_s = __shfl_down(ou_s, 1), rv_s = __shfl_down(ov_s, 1);
       T ru_c = __shfl_down(ou_c, 1), rv_c = __shfl_down(ov_c, 1);
-      for (int j = j0; j < j1; ++j) {
+      // Row j+1 enters from slot S = (j + 1 - j0) mod D; the faces of row j follow.
+      auto row = [&](auto slot, int j) {
+        constexpr int S = decltype(slot)::value;
         T ou_n, ov_n;   // row j+1
-        prim(q1, ou_n, ov_n);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) q1[k] = q2[k];
-        fetch(j + 3, q2);
+        prim(j + 1, ring[S], ou_n, ov_n);
+        fetch(j + 1 + D, ring[S]);
         const T lu_n = __shfl_up(ou_n, 1), lv_n = __shfl_up(ov_n, 1);
         const T ru_n = __shfl_down(ou_n, 1), rv_n = __shfl_down(ov_n, 1);
         if (out) {
@@ -169,7 +252,18 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
         ou_s = ou_c; ov_s = ov_c; ou_c = ou_n; ov_c = ov_n;
         lu_s = lu_c; lv_s = lv_c; lu_c = lu_n; lv_c = lv_n;
         ru_s = ru_c; rv_s = rv_c; ru_c = ru_n; rv_c = rv_n;
+      };
+      static_assert(D == 4, "the round below is written for a 4-row ring");
+      int j = j0;
+      for (; j + D <= j1; j += D) {
+        row(IC<1>{}, j);
+        row(IC<2>{}, j + 1);
+        row(IC<3>{}, j + 2);
+        row(IC<0>{}, j + 3);
       }
+      if (j < j1) row(IC<1>{}, j);
+      if (j + 1 < j1) row(IC<2>{}, j + 1);
+      if (j + 2 < j1) row(IC<3>{}, j + 2);
     }
   }
   unsigned long long kmin = cfl.key();

@@ -423,10 +423,14 @@ int ctx_scratch(jx_ctx* c, size_t state_bytes, size_t nblocks, bool kick, StepSc
 
 // Launch the RK stages of one step.  `first`: the saved register is the caller's input
 // (floored on read).  The result lands in `A`.  `raw`: kick the final stage.  `want_dtkey`:
-// the step's last kernel reduces the next CFL limit.
+// the step's last kernel reduces the next CFL limit.  `kick_ts` (with `raw`): the kick runs
+// inside the next step's timestep reduction (timestep_kernel<T, true> with these
+// arguments), which reads the last stage's output from a scratch register and writes A.
 template <typename T>
 int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const T* in, T* A,
-                bool first, const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr) {
+                bool first, const T* raw, bool want_dtkey, const SgsHook<T>* hook = nullptr,
+                const jx::TimestepArgs<T>* kick_ts = nullptr) {
+  const bool fused_kick = raw && kick_ts;
   const jx_run_params* p = s.p;
   const RkTable tab = rk_table(p);
   const int ns = tab.n;
@@ -457,7 +461,10 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
     const bool last = (k == ns - 1);
     a.stage1 = (k == 0 && plain_first_row(r)) ? 1 : 0;
     a.u_in = prev;
-    a.u_out = (last && prev != A) ? A : ((k & 1) ? C : B);
+    if (last && fused_kick)
+      a.u_out = (prev == B) ? C : B;   // the register the last stage does not read
+    else
+      a.u_out = (last && prev != A) ? A : ((k & 1) ? C : B);
     a.u_sv = a.stage1 ? nullptr : (saved_in_s ? S : src);
     a.floor_saved = (!a.stage1 && !saved_in_s && first) ? 1 : 0;
     a.s_out = r.has_delta ? S : nullptr;
@@ -497,10 +504,11 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
                                       s.bx, c->stream, ev ? ev->first : nullptr,
                                       ev ? ev->second : nullptr);
     if (e != hipSuccess) return hip_fail(c, e, "stage kernel launch");
-    if (last && a.u_out != A)
+    if (last && a.u_out != A && !fused_kick)
       JX_HIP(c, hipMemcpyAsync(A, a.u_out, (size_t)p->nx * p->ny * p->nvar * sizeof(T),
                                hipMemcpyDeviceToDevice, c->stream));
   }
+  const T* kick_src = prev;   // the last stage's output
   if (raw) {
     // Kick: forcing.py:137-149 then hydro.py:132, on the post-drag state in A.
     const size_t n = (size_t)p->nx * p->ny;
@@ -508,7 +516,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
     hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
                        w.partial, (int)nblocks, one_pass ? 2 : 0, (double)n);
     jx::KickArgs<T> ka{};
-    ka.u = A;
+    ka.u = fused_kick ? const_cast<T*>(kick_src) : A;   // the moment pass reads it only
     ka.raw = raw;
     ka.nx = p->nx;
     ka.ny = p->ny;
@@ -526,8 +534,19 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
       hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
                          w.partial, (int)g.x, 1, (double)n);
     }
-    ka.dtkey = want_dtkey ? &c->ctrl->dtkey : nullptr;
-    hipLaunchKernelGGL(jx::kick_apply_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
+    if (fused_kick) {
+      jx::TimestepArgs<T> t = *kick_ts;
+      t.q = kick_src;
+      t.primitive = 0;
+      t.raw = raw;
+      t.out = A;
+      t.waves = jx::kTsKickWaves;
+      hipLaunchKernelGGL((jx::timestep_kernel<T, true>), jx::ts_grid(p->nx, p->ny, t.waves),
+                         dim3(jx::kAuxBlock), 0, c->stream, t);
+    } else {
+      ka.dtkey = want_dtkey ? &c->ctrl->dtkey : nullptr;
+      hipLaunchKernelGGL(jx::kick_apply_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
+    }
   }
   JX_HIP(c, hipGetLastError());
   return JX_OK;
@@ -553,6 +572,7 @@ jx::TimestepArgs<T> timestep_args(jx_ctx* c, const jx_run_params* p, const T* q,
   t.ctrl = ctrl;
   t.dtkey = &c->ctrl->dtkey;
   t.nukey = &c->ctrl->nukey;
+  t.waves = jx::kTsWaves;
   return t;
 }
 
@@ -684,9 +704,11 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     if (open)
       hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
     if (forced) launch_render<T>(c, ra);
-    int r = launch_step<T>(c, s, w, in, out, first, raw, !smag, hook);
+    // Forced runs with Smagorinsky kick inside the next timestep reduction.
+    const bool fuse = forced && smag;
+    int r = launch_step<T>(c, s, w, in, out, first, raw, !smag, hook, fuse ? &ts : nullptr);
     if (r) return r;
-    if (smag)
+    if (smag && !fuse)
       hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
                          c->stream, ts);
     return JX_OK;
