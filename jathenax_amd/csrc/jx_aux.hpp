@@ -71,8 +71,8 @@ __host__ __device__ inline TsTiling ts_tiling(int nx, int ny, int waves = 8192) 
 }
 inline dim3 ts_grid(int nx, int ny, int waves = 8192) {
   const TsTiling t = ts_tiling(nx, ny, waves);
-  const int waves = t.nstrips * t.nbands;
-  return dim3((unsigned)((waves + kAuxBlock / 64 - 1) / (kAuxBlock / 64)));
+  const int used = t.nstrips * t.nbands;
+  return dim3((unsigned)((used + kAuxBlock / 64 - 1) / (kAuxBlock / 64)));
 }
 
 // Each lane marches up its column. A cell's primitives are recovered once, when its row
