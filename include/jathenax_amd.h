@@ -88,6 +88,12 @@ int jx_get_tiles(jx_ctx* ctx, const jx_run_params* p, int* block_x, int* rows);
  * returns the number of timed launches and their summed duration, then resets. */
 int jx_set_kernel_timing(jx_ctx* ctx, int on);
 int jx_kernel_timing(jx_ctx* ctx, int* launches, double* total_ms);
+/* FP32 forced runs with Smagorinsky evaluate the raw force per cell where the kick reads
+ * it (from per-row sums of the basis's mode groups, at most 4 groups) instead of rendering
+ * the (ny, nx, 2) array each step; the values are the rendered ones bit for bit.  On by
+ * default; off renders the array.  No reference counterpart: a performance switch, whose
+ * two settings give the same results. */
+int jx_set_fused_forcing(jx_ctx* ctx, int on);
 
 /* One step, all RK stages (replaces isothermal_step, jathenax/hydro.py:92-146).
  * U_in is not modified; the floored result goes to U_out (distinct from U_in).

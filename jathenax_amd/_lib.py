@@ -46,6 +46,7 @@ SIGNATURES = {
     "jx_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "jx_get_tiles": (c_int, [c_void_p, POINTER(RunParams), POINTER(c_int), POINTER(c_int)]),
     "jx_set_kernel_timing": (c_int, [c_void_p, c_int]),
+    "jx_set_fused_forcing": (c_int, [c_void_p, c_int]),
     "jx_kernel_timing": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double)]),
     "jx_isothermal_step": (c_int, [c_void_p, POINTER(RunParams), c_void_p, c_void_p, c_double,
                                    c_void_p, c_double]),

@@ -41,10 +41,12 @@ template <typename T> struct TimestepArgs {
   unsigned long long* dtkey;      // min slot
   unsigned long long* nukey;      // max slot
   // timestep_kernel<T, true> only: q holds the post-stage state of a forced step, raw its
-  // raw force, and the kicked state goes to out (another buffer than q).
+  // raw force (or force.pq its per-row group sums), and the kicked state goes to out
+  // (another buffer than q).
   const T* raw;
   T* out;
   int waves;                      // target wave count of the tiling (ts_tiling)
+  ForceEval<T> force;
 };
 
 __device__ __forceinline__ unsigned long long max_key_of(double x) {
@@ -114,6 +116,10 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
     if (xc < 0) xc += nx;
     const bool out = lane >= 1 && lane <= kTsCols && x0 + lane - 1 < nx;
     const int j0 = band * tl.rows, j1 = min(j0 + tl.rows, ny);
+    ForceCols<T> fc{};
+    if constexpr (KICK) {
+      if (a.force.pq) fc = force_cols(a.force, nx, xc);
+    }
     // Raw values of a row (periodic): the state, and with KICK the raw force and the
     // passive variables (nvar <= 5) that the kicked copy carries.
     struct Row {
@@ -142,8 +148,11 @@ __global__ void __launch_bounds__(kAuxBlock) timestep_kernel(TimestepArgs<T> a) 
           w.x[0] = src[3];
           w.x[1] = nv > 4 ? src[4] : T(0);
         }
-        row_load<T, 2>(row_rsrc(a.raw + (size_t)j * nx * 2, nx * 2 * (int)sizeof(T)),
-                       (unsigned)(xc * 2 * (int)sizeof(T)), w.f);
+        if (a.force.pq)
+          force_at(a.force, fc, j, w.f[0], w.f[1]);
+        else
+          row_load<T, 2>(row_rsrc(a.raw + (size_t)j * nx * 2, nx * 2 * (int)sizeof(T)),
+                         (unsigned)(xc * 2 * (int)sizeof(T)), w.f);
       } else {
         w.q[0] = src[0];
         w.q[1] = src[1];
@@ -507,6 +516,51 @@ static __global__ void mode_groups_kernel(int M, int* grp) {
 
 constexpr int kGroupChunk = 32;
 
+// Row j's sums of group g (P0, Q0, P1, Q1): the y factors of the group's modes with their
+// OU amplitudes (NaN when `fill`: no OU row in range), each mode's sx sign folded into Q.
+__device__ __forceinline__ float4 group_row_sums(const RenderArgs<float>& a, const int* grp,
+                                                 const float* amp, bool fill, int g, int j) {
+  const int M = a.n_modes, ny = a.ny;
+  const int* gstart = grp + 1;
+  const int* order = gstart + (M + 1);
+  const int* sign = order + M;
+  const float nan = __builtin_nanf("");
+  float p0 = 0.0f, q0 = 0.0f, p1 = 0.0f, q1 = 0.0f;
+  for (int k = gstart[g]; k < gstart[g + 1]; ++k) {
+    const int m = order[k];
+    const float s = (float)sign[m];
+    const float cyv = a.cy[(size_t)m * ny + j], syv = a.sy[(size_t)m * ny + j];
+    const float a00 = fill ? nan : amp[m * 4 + 0], a01 = fill ? nan : amp[m * 4 + 1];
+    const float a10 = fill ? nan : amp[m * 4 + 2], a11 = fill ? nan : amp[m * 4 + 3];
+    p0 = fmaf(cyv, a00, fmaf(-syv, a01, p0));
+    q0 = fmaf(s, fmaf(syv, a00, cyv * a01), q0);
+    p1 = fmaf(cyv, a10, fmaf(-syv, a11, p1));
+    q1 = fmaf(s, fmaf(syv, a10, cyv * a11), q1);
+  }
+  return make_float4(p0, q0, p1, q1);
+}
+
+// Per-row group sums for force_at (at most kMaxForceGroups groups; the rest of a row's
+// slots are zero): one thread per (group slot, row), rows fastest so that the basis rows
+// cy, sy are read coalesced.  With a control block, only a step that opens a new OU row
+// recomputes them, as the render.
+static __global__ void __launch_bounds__(kAuxBlock) force_rows_kernel(RenderArgs<float> a,
+                                                                      const int* grp,
+                                                                      float4* pq) {
+  int row = a.row;
+  if (a.ctrl) {
+    if (a.ctrl->done || !a.ctrl->render) return;
+    row = a.ctrl->update;
+  }
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.ny * kMaxForceGroups) return;
+  const int g = e / a.ny, j = e - g * a.ny;
+  const bool fill = (row < 0) || (row >= a.n_updates);
+  const float* amp = a.table + (size_t)(fill ? 0 : row) * a.n_modes * 4;
+  pq[(size_t)j * kMaxForceGroups + g] = (g < grp[0]) ? group_row_sums(a, grp, amp, fill, g, j)
+                                                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // Blocks as render_force_kernel: kAuxBlock columns by render_rows_per_block rows.
 static __global__ void __launch_bounds__(kAuxBlock) render_grouped_kernel(RenderArgs<float> a,
                                                                    const int* grp) {
@@ -519,11 +573,9 @@ static __global__ void __launch_bounds__(kAuxBlock) render_grouped_kernel(Render
   const int M = a.n_modes, nx = a.nx, ny = a.ny;
   const bool fill = (row < 0) || (row >= a.n_updates);
   const float* amp = a.table + (size_t)(fill ? 0 : row) * M * 4;
-  const float nan = __builtin_nanf("");
   const int G = grp[0];
   const int* gstart = grp + 1;
   const int* order = gstart + (M + 1);
-  const int* sign = order + M;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = i < nx;
   const int rb = render_rows_per_block(nx, ny);
@@ -538,19 +590,7 @@ static __global__ void __launch_bounds__(kAuxBlock) render_grouped_kernel(Render
     for (int e = threadIdx.x; e < gc * rb; e += blockDim.x) {
       const int q = e / rb, r = e - q * rb;
       const int j = y0 + min(r, nr - 1);
-      float p0 = 0.0f, q0 = 0.0f, p1 = 0.0f, q1 = 0.0f;
-      for (int k = gstart[g0 + q]; k < gstart[g0 + q + 1]; ++k) {
-        const int m = order[k];
-        const float s = (float)sign[m];
-        const float cyv = a.cy[(size_t)m * ny + j], syv = a.sy[(size_t)m * ny + j];
-        const float a00 = fill ? nan : amp[m * 4 + 0], a01 = fill ? nan : amp[m * 4 + 1];
-        const float a10 = fill ? nan : amp[m * 4 + 2], a11 = fill ? nan : amp[m * 4 + 3];
-        p0 = fmaf(cyv, a00, fmaf(-syv, a01, p0));
-        q0 = fmaf(s, fmaf(syv, a00, cyv * a01), q0);
-        p1 = fmaf(cyv, a10, fmaf(-syv, a11, p1));
-        q1 = fmaf(s, fmaf(syv, a10, cyv * a11), q1);
-      }
-      pq[r][q] = make_float4(p0, q0, p1, q1);
+      pq[r][q] = group_row_sums(a, grp, amp, fill, g0 + q, j);
     }
     __syncthreads();
     for (int q = 0; q < gc; ++q) {

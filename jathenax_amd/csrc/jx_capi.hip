@@ -68,6 +68,9 @@ struct jx_ctx {
   size_t raw_bytes = 0;
   void* groups = nullptr;     // FP32 render: mode groups of the forcing basis
   size_t groups_bytes = 0;
+  void* force_pq = nullptr;   // FP32 evaluated forcing: per-row group sums (ny, 4, 4)
+  size_t force_pq_bytes = 0;
+  bool fused_forcing = true;  // FP32 forced Smagorinsky runs evaluate the force in place
   double* partial = nullptr;  // per-block partial sums
   size_t partial_count = 0;
   double* sums = nullptr;     // small device result vector
@@ -184,13 +187,26 @@ template <typename T> int prepare_render(jx_ctx* c, const jx::RenderArgs<T>& ra)
   return JX_OK;
 }
 
-template <typename T> void launch_render(jx_ctx* c, const jx::RenderArgs<T>& ra) {
-  if constexpr (std::is_same<T, float>::value)
+size_t force_pq_bytes(int ny) { return (size_t)ny * jx::kMaxForceGroups * 4 * sizeof(float); }
+
+// `eval`: the run evaluates the force where it is used (FP32): only the per-row group sums
+// are computed.
+template <typename T> void launch_render(jx_ctx* c, const jx::RenderArgs<T>& ra,
+                                         bool eval = false) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (eval) {
+      const unsigned nt = (unsigned)ra.ny * jx::kMaxForceGroups;
+      hipLaunchKernelGGL(jx::force_rows_kernel, dim3((nt + jx::kAuxBlock - 1) / jx::kAuxBlock),
+                         dim3(jx::kAuxBlock), 0, c->stream, ra,
+                         static_cast<const int*>(c->groups), static_cast<float4*>(c->force_pq));
+      return;
+    }
     hipLaunchKernelGGL(jx::render_grouped_kernel, render_grid(ra.nx, ra.ny), dim3(jx::kAuxBlock),
                        0, c->stream, ra, static_cast<const int*>(c->groups));
-  else
+  } else {
     hipLaunchKernelGGL(jx::render_force_kernel<T>, render_grid(ra.nx, ra.ny),
                        dim3(jx::kAuxBlock), 0, c->stream, ra);
+  }
 }
 
 dim3 aux_grid(size_t n) {
@@ -624,6 +640,33 @@ int step_noalloc_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, do
   return launch_step<T>(c, s, w, in, out, true, raw, false, nullptr);
 }
 
+// FP32 forced runs with Smagorinsky (their final stage takes the kick moments in one pass
+// and the kick runs inside the timestep reduction, the only two readers of the force)
+// evaluate the force where it is used when the basis has at most kMaxForceGroups mode
+// groups: the plan's stages and `*fe` get the per-row group sums (c->force_pq).  Reads the
+// group count back once (the groups are found on the device by prepare_render).
+template <typename T>
+int prepare_force_eval(jx_ctx* c, const jx_run_params* p, const jx_forcing* f, bool smag,
+                       StepPlan<T>* s, jx::ForceEval<T>* fe) {
+  *fe = jx::ForceEval<T>{};
+  if constexpr (std::is_same<T, float>::value) {
+    if (!smag || !c->fused_forcing) return JX_OK;
+    int* g = reinterpret_cast<int*>(c->host_sums);
+    JX_HIP(c, hipMemcpyAsync(g, c->groups, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    JX_HIP(c, hipStreamSynchronize(c->stream));
+    if (*g < 1 || *g > jx::kMaxForceGroups) return JX_OK;
+    int rc = ensure(c, &c->force_pq, &c->force_pq_bytes, force_pq_bytes(p->ny));
+    if (rc) return rc;
+    fe->pq = static_cast<const float*>(c->force_pq);
+    fe->cx = static_cast<const float*>(f->cos_x);
+    fe->sx = static_cast<const float*>(f->sin_x);
+    fe->grp = static_cast<const int*>(c->groups);
+    fe->n_modes = f->n_modes;
+    s->base.force = *fe;
+  }
+  return JX_OK;
+}
+
 template <typename T>
 int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx_forcing* f,
                 double* t_out, double* dt_out, int* cyc_out, const SgsHook<T>* hook = nullptr,
@@ -655,6 +698,7 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     return JX_OK;
   }
   const bool smag = p->smagorinsky_cs > 0.0;
+  jx::ForceEval<T> ts_force{};   // pq set: the force is evaluated where it is used
   jx::LoopArgs la{};
   la.tlim = p->tlim;
   la.cfl = p->cfl;
@@ -683,27 +727,34 @@ int evolve_impl(jx_ctx* c, const jx_run_params* p, const T* in, T* out, const jx
     ra.ctrl = c->ctrl;
     rc = prepare_render<T>(c, ra);
     if (rc) return rc;
-    JX_HIP(c, hipMemsetAsync(c->raw, 0, n * 2 * sizeof(T), c->stream));
+    rc = prepare_force_eval<T>(c, p, f, smag, &s, &ts_force);
+    if (rc) return rc;
+    if (ts_force.pq)
+      JX_HIP(c, hipMemsetAsync(c->force_pq, 0, force_pq_bytes(p->ny), c->stream));
+    else
+      JX_HIP(c, hipMemsetAsync(c->raw, 0, n * 2 * sizeof(T), c->stream));
     if (s0.update >= 0) {
       // Resume: the raw force of the OU row in effect (forcing.py:111-122).
       jx::RenderArgs<T> r0 = ra;
       r0.ctrl = nullptr;
       r0.row = s0.update;
-      launch_render<T>(c, r0);
+      launch_render<T>(c, r0, ts_force.pq != nullptr);
     }
   }
+  const bool eval_force = ts_force.pq != nullptr;
   const T* raw = forced ? static_cast<const T*>(c->raw) : nullptr;
   // Initial timestep from the caller's state (run.py:42-46).  Each step opens with the
   // one-thread begin_step kernel (commit the last step, test the loop condition, pick dt).
   hipLaunchKernelGGL(jx::timestep_kernel<T>, jx::ts_grid(p->nx, p->ny), dim3(jx::kAuxBlock), 0,
                      c->stream, timestep_args<T>(c, p, in, 0, c->ctrl));
-  const jx::TimestepArgs<T> ts = timestep_args<T>(c, p, out, 0, c->ctrl);
+  jx::TimestepArgs<T> ts = timestep_args<T>(c, p, out, 0, c->ctrl);
+  ts.force = ts_force;
   // `open`: launch the step's begin_step (false for the first step after a chunk boundary,
   // which the boundary's begin_step has opened already).
   auto one_step = [&](bool first, bool open) -> int {
     if (open)
       hipLaunchKernelGGL(jx::begin_step_kernel, dim3(1), dim3(1), 0, c->stream, c->ctrl, la);
-    if (forced) launch_render<T>(c, ra);
+    if (forced) launch_render<T>(c, ra, eval_force);
     // Forced runs with Smagorinsky kick inside the next timestep reduction.
     const bool fuse = forced && smag;
     int r = launch_step<T>(c, s, w, in, out, first, raw, !smag, hook, fuse ? &ts : nullptr);
@@ -910,6 +961,7 @@ void jx_ctx_destroy(jx_ctx* c) {
     if (b) (void)hipFree(b);
   if (c->raw) (void)hipFree(c->raw);
   if (c->groups) (void)hipFree(c->groups);
+  if (c->force_pq) (void)hipFree(c->force_pq);
   if (c->partial) (void)hipFree(c->partial);
   if (c->ctrl) (void)hipFree(c->ctrl);
   if (c->sums) (void)hipFree(c->sums);
@@ -942,6 +994,12 @@ int jx_get_tiles(jx_ctx* c, const jx_run_params* p, int* block_x, int* rows) {
   const bool exact = is_pow2(p->dx) && is_pow2(p->dy) && is_pow2(p->cs);
   if (p->dtype == 1) pick_tiles<float>(c, p, visc, exact, block_x, rows);
   else pick_tiles<double>(c, p, visc, exact, block_x, rows);
+  return JX_OK;
+}
+
+int jx_set_fused_forcing(jx_ctx* c, int on) {
+  if (!c) return JX_EINVAL;
+  c->fused_forcing = on != 0;
   return JX_OK;
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "jx_types.hpp"
 
@@ -682,5 +683,72 @@ template <> struct CflMin<float> {
     return smax == 0.0f ? ~0ull : key_of(1.0 / (double)smax);
   }
 };
+
+// ---------------------------------------------------------------------------------------
+// FP32 raw force evaluated where it is used (forcing.py:111-122).  The grouped render
+// writes, per component, f(j, i) = sum over groups g of cx_g(i) P_g(j) - sx_g(i) Q_g(j), for
+// groups of modes that share their x basis rows (up to the sign of sx).  With at most
+// kMaxForceGroups groups only the per-row sums (P0, Q0, P1, Q1) are stored
+// (force_rows_kernel, (ny, kMaxForceGroups, 4) values), and a kernel that needs the force
+// evaluates it per cell from its column's x rows in the render's operation order: the same
+// values, bit for bit, without the (ny, nx, 2) array.
+// ---------------------------------------------------------------------------------------
+
+constexpr int kMaxForceGroups = 4;
+
+template <typename T> struct ForceEval {
+  const T* pq;                 // per-row group sums; nullptr: the kernel reads a raw array
+  const T* cx; const T* sx;    // basis x rows (M, nx)
+  const int* grp;              // mode groups: G, gstart[M+1], order[M], ...
+  int n_modes;
+};
+
+// Column i's x rows of each group's first mode (zero past the group count G).
+template <typename T> struct ForceCols {
+  T cx[kMaxForceGroups], sx[kMaxForceGroups];
+  int G;
+};
+
+// The group table and the row sums are not written while a kernel reads them, and every
+// lane of a wave reads the same words: they are read through the constant address space
+// (scalar loads).
+template <typename T>
+__device__ __forceinline__ ForceCols<T> force_cols(const ForceEval<T>& f, int nx, int i) {
+  ForceCols<T> c;
+  const cmem_t<int>* grp = (const cmem_t<int>*)f.grp;
+  c.G = grp[0];
+  const cmem_t<int>* gstart = grp + 1;
+  const cmem_t<int>* order = gstart + (f.n_modes + 1);
+#pragma unroll
+  for (int g = 0; g < kMaxForceGroups; ++g) {
+    c.cx[g] = c.sx[g] = T(0);
+    if (g < c.G) {
+      const size_t m = (size_t)order[gstart[g]];
+      c.cx[g] = f.cx[m * nx + i];
+      c.sx[g] = f.sx[m * nx + i];
+    }
+  }
+  return c;
+}
+
+template <typename T> __device__ __forceinline__ T fma_of(T a, T b, T c) {
+  if constexpr (std::is_same<T, float>::value) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
+// The raw force of row j at the column of `c`: render_grouped_kernel's accumulation.
+template <typename T>
+__device__ __forceinline__ void force_at(const ForceEval<T>& f, const ForceCols<T>& c, int j,
+                                         T& f0, T& f1) {
+  const cmem_t<T>* row = (const cmem_t<T>*)f.pq + (size_t)j * (kMaxForceGroups * 4);
+  f0 = f1 = T(0);
+#pragma unroll
+  for (int g = 0; g < kMaxForceGroups; ++g) {
+    if (g < c.G) {
+      f0 = fma_of(c.cx[g], row[4 * g + 0], fma_of(-c.sx[g], row[4 * g + 1], f0));
+      f1 = fma_of(c.cx[g], row[4 * g + 2], fma_of(-c.sx[g], row[4 * g + 3], f1));
+    }
+  }
+}
 
 }  // namespace jx

@@ -33,6 +33,8 @@ template <typename T> struct StageArgs {
   T* u_out;                     // stage output
   T* s_out;                     // RK4 saved register output (nullptr when no delta)
   const T* __restrict__ raw;    // raw force (ny, nx, 2) when this stage feeds the kick
+  ForceEval<T> force;           // FP32 viscous kernels: force.pq set, the force is evaluated
+                                // per cell instead of read from raw
   const T* __restrict__ tau;    // learned stress (ny, nx, 3) of this stage, or nullptr
                                 // (VISC kernels only)
   const Ctrl* ctrl;             // dt of the step and the done flag
@@ -252,6 +254,11 @@ stage_kernel(StageArgs<T> args) {
   double acc[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) acc[k] = 0.0;
+  // Kick moments from the evaluated force: this lane's column factors of each mode group.
+  ForceCols<T> fcol{};
+  if constexpr (KS == 7) {
+    if (a.partial && a.force.pq) fcol = force_cols(a.force, nx, xc);
+  }
   CflMin<T> cfl(a.dx, a.dy, a.cs);
   // Loads run ahead of their use so that HBM latency under full load (longer than one row
   // iteration) stays hidden.  With the 4-row ring (even, so the loop parity is known at
@@ -566,10 +573,17 @@ stage_kernel(StageArgs<T> args) {
         }
         if (a.partial) {
           // The kick normalises against the post-drag state; the floor comes after it.
-          const T* rf = a.raw + (size_t)j * nx * 2 + (x0 + t - 3) * 2;
           if constexpr (KS == 7) {
+            T f0, f1;
+            if (a.force.pq) {
+              force_at(a.force, fcol, j, f0, f1);
+            } else {
+              const T* rf = a.raw + (size_t)j * nx * 2 + (x0 + t - 3) * 2;
+              f0 = rf[0];
+              f1 = rf[1];
+            }
             // float products are exact in float64
-            const double rho = un[0], r0 = rf[0], r1 = rf[1];
+            const double rho = un[0], r0 = f0, r1 = f1;
             const double m0 = un[1], m1 = un[2];
             acc[0] += rho;
             acc[1] += rho * r0;
@@ -579,6 +593,7 @@ stage_kernel(StageArgs<T> args) {
             acc[5] += m0;
             acc[6] += m1;
           } else {
+            const T* rf = a.raw + (size_t)j * nx * 2 + (x0 + t - 3) * 2;
             acc[0] += (double)un[0];
             acc[1] += (double)(un[0] * rf[0]);
             acc[2] += (double)(un[0] * rf[1]);

@@ -62,6 +62,11 @@ class Context:
     def check(self, rc, what=""):
         _lib.check(rc, self.handle, what)
 
+    def set_fused_forcing(self, on):
+        """FP32 forced Smagorinsky runs: evaluate the force where the kick reads it (True,
+        the default) or render the raw force array every step (False); same results."""
+        self.check(_lib.load().jx_set_fused_forcing(self.handle, int(bool(on))), "forcing")
+
     def set_kernel_timing(self, on):
         self.check(_lib.load().jx_set_kernel_timing(self.handle, int(bool(on))), "timing")
 

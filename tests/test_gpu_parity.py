@@ -319,6 +319,33 @@ def test_fp32_driven_run_tracks_fp64(jx, viscosity):
     assert np.max(np.abs(U32 - U64)) <= 2e-5 * np.max(np.abs(U64))
 
 
+@pytest.mark.parametrize("shape,nlim", [((48, 40), 24), ((300, 173), 6)])
+def test_fp32_evaluated_forcing_equals_rendered(jx, shape, nlim):
+    """FP32 forced Smagorinsky runs (the driven-turbulence configuration) evaluate the raw
+    force where the kick reads it, from per-row sums of the basis's mode groups; rendering
+    the (ny, nx, 2) array every step instead must give the same run bit for bit."""
+    from jathenax_amd import workloads
+    from jathenax_amd.device import context_for
+    nx, ny = shape
+    s = workloads.turbulence_spec(nx, ny, 0.5, 1618, 0.002, "rk2", "wenoz", "hlle", cfl=0.3,
+                                  dtype=np.float32)
+    s["kw"]["dedt"] = 0.5
+    U0 = torch.tensor(s["U"], dtype=torch.float32, device="cuda")
+    args = (s["dx"], s["dy"], s["cs"], s["dfloor"], s["integrator"], s["cfl"], s["tlim"], nlim)
+    ctx = context_for(U0)
+    out = {}
+    try:
+        for fused in (True, False):
+            ctx.set_fused_forcing(fused)
+            out[fused] = jx.evolve_isothermal(U0, *args, **s["kw"])
+    finally:
+        ctx.set_fused_forcing(True)
+    assert out[True][3] == out[False][3] == nlim
+    assert out[True][1:] == out[False][1:]
+    np.testing.assert_array_equal(_np(out[True][0]), _np(out[False][0]))
+    assert float(torch.max(torch.abs(out[True][0][..., 1]))) > 0.0   # the force acted
+
+
 def test_fp32_tracks_fp64(jx):
     from jathenax_amd import problems
     U0 = problems.smooth_wave(128, 128)
