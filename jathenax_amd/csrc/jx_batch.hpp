@@ -74,7 +74,16 @@ enum { CNT_STAGE = 0, CNT_MOM, CNT_APPLY, CNT_TSTEP, CNT_TOTAL };
 // this round and the prefix sums of their tile counts, so worker waves only visit the
 // tiles that have work.  Layout of list L at lists + L * list_stride(cap):
 //   [0] entries n, [1] total tiles, [2 .. 2+n] prefix (n+1 values), then n instance ids.
-enum { WL_STAGE = 0, WL_KICK, WL_SMAG, WL_TOTAL };
+// WL_KICK: instances ending a forced step; WL_SMAG: Smagorinsky instances ending a step
+// (their kick, when forced, runs inside the timestep pass); WL_APPLY: forced instances
+// without Smagorinsky, whose kick is a pass of its own.
+enum { WL_STAGE = 0, WL_KICK, WL_SMAG, WL_APPLY, WL_TOTAL };
+
+// Kick sums per (strip, 8-row group): FP32 takes rho, rho r, rho |r|^2, m.r and m in the
+// final stage's own pass (forcing.py:141-145 expanded about the mean, summed in float64);
+// FP64 takes rho and rho r there and the second moments in a pass of their own, in the
+// reference's form.
+template <typename T> constexpr int batch_kick_sums() { return Fast<T>::value ? 7 : 3; }
 
 __host__ __device__ inline int list_stride(int cap) { return 2 + (cap + 1) + cap; }
 
@@ -117,6 +126,14 @@ __device__ __forceinline__ Tile tile_of(const BatchInst<T>& I, int rows, int q) 
   t.y0 = (q / I.nstrips) * rows;
   t.rows = min(rows, I.ny - t.y0);
   return t;
+}
+
+// Register (0 = A) that holds a forced instance's final-stage output before its kick: a
+// forced Smagorinsky instance's final stage writes a register that stage does not read, and
+// the timestep pass kicks it into A; other forced instances are kicked in A in place.
+//   rk2: B|A -> C   rk3: C|A -> B   rk4: B|S -> C
+template <typename T> __device__ __forceinline__ int kick_reg(const BatchInst<T>& I) {
+  return (I.forced && I.smag_on) ? (I.nstages == 3 ? 1 : 2) : 0;
 }
 
 // Fixed-order butterfly sum over the wave of three doubles.
@@ -252,6 +269,7 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     sv_r = from_a ? 0 : 3;
     floor_saved = from_a;
     has_sv = true;
+    if (k == ns - 1) out_r = kick_reg(I);
   }
   gmem_t<T>* const regs = gmem(I.A);
   const gmem_t<T>* g_in = regs + (size_t)in_r * I.reg_stride;
@@ -292,7 +310,12 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
     for (int v = 0; v < NV; ++v)
       pf[v] = src[v];
   }
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  // Kick sums of the current 8-row group: rho and rho r in float64; FP32's four one-pass
+  // sums (rho |r|^2, m.r, m) in float within the group (at most 8 terms per lane), which
+  // keeps the tile within three waves per SIMD.
+  constexpr int KS = batch_kick_sums<T>();
+  double acc[3] = {0.0, 0.0, 0.0};
+  float acc2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   CflMin<T> cfl(I.dx, I.dy, I.cs);
 
   auto body = [&](auto pc, int i) {
@@ -518,9 +541,22 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
         }
         if (kick) {
           const gmem_t<T>* rf = g_raw + cell * 2;
-          acc0 += (double)un[0];
-          acc1 += (double)(un[0] * rf[0]);
-          acc2 += (double)(un[0] * rf[1]);
+          if constexpr (KS == 7) {
+            // float products are exact in float64
+            const double rho = un[0], r0 = rf[0], r1 = rf[1];
+            acc[0] += rho;
+            acc[1] += rho * r0;
+            acc[2] += rho * r1;
+            const float f0 = rf[0], f1 = rf[1];
+            acc2[0] += un[0] * (f0 * f0 + f1 * f1);
+            acc2[1] += un[1] * f0 + un[2] * f1;
+            acc2[2] += un[1];
+            acc2[3] += un[2];
+          } else {
+            acc[0] += (double)un[0];
+            acc[1] += (double)(un[0] * rf[0]);
+            acc[2] += (double)(un[0] * rf[1]);
+          }
         } else {
           un[0] = floor_to(un[0], dfloor);
           if (want_key) {
@@ -535,14 +571,21 @@ __device__ __forceinline__ void batch_stage_tile(const BatchInst<T>& I, BatchCtr
       // Kick sums close at the end of every 8-row group (bands start on group bounds).
       const int j = r - 3;
       if (kick && (((j & (kGroupRows - 1)) == kGroupRows - 1) || j == yend - 1)) {
-        wave_sum3(acc0, acc1, acc2);
-        if (t == 0) {
-          gmem_t<double>* g = g_part + 3 * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
-          g[0] = acc0;
-          g[1] = acc1;
-          g[2] = acc2;
+        double all[KS];
+#pragma unroll
+        for (int q = 0; q < KS; ++q) all[q] = (q < 3) ? acc[q] : (double)acc2[q < 3 ? 0 : q - 3];
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+          for (int q = 0; q < KS; ++q) all[q] += __shfl_xor(all[q], off);
         }
-        acc0 = acc1 = acc2 = 0.0;
+        if (t == 0) {
+          gmem_t<double>* g = g_part + KS * ((size_t)tl.strip * I.ngroups + j / kGroupRows);
+#pragma unroll
+          for (int q = 0; q < KS; ++q) g[q] = all[q];
+        }
+        acc[0] = acc[1] = acc[2] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc2[q] = 0.0f;
       }
     }
     // gn is set from iteration 5 on and gp first read at iteration 6.
@@ -634,7 +677,8 @@ __global__ void __launch_bounds__(kCtrlBlock) batch_control_kernel(BatchTabs<T> 
       if (C.active) {
         const bool last = C.stage == I.nstages - 1;
         f = (1 << WL_STAGE) | ((last && I.forced) ? 1 << WL_KICK : 0) |
-            ((last && I.smag_on) ? 1 << WL_SMAG : 0);
+            ((last && I.smag_on) ? 1 << WL_SMAG : 0) |
+            ((last && I.forced && !I.smag_on) ? 1 << WL_APPLY : 0);
         work += (long long)I.nstrips * I.ny;
       }
     } else {
@@ -739,8 +783,8 @@ __device__ __forceinline__ bool kicking(const BatchCtrl& C, int forced, int nsta
 }
 
 // Per-instance fixed-order sums of the group partials, one wave per instance slot: lane l
-// adds groups l, l + 64, ... in order, then a fixed butterfly.  mode 0: mean force;
-// mode 1: scale.
+// adds groups l, l + 64, ... in order, then a fixed butterfly.  mode 0: mean force (FP32:
+// and the scale, from the one-pass sums); mode 1 (FP64): scale from the second moments.
 template <typename T>
 __global__ void __launch_bounds__(256) batch_kick_reduce_kernel(BatchTabs<T> b, int mode) {
   const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -751,21 +795,36 @@ __global__ void __launch_bounds__(256) batch_kick_reduce_kernel(BatchTabs<T> b, 
   if (!kicking(C, I.forced, I.nstages)) return;
   const int ng = I.nstrips * I.ngroups;
   const gmem_t<double>* part = gmem(static_cast<const double*>(I.part));
-  double a0 = 0, a1 = 0, a2 = 0;
+  // FP64 second moments (mode 1) come in threes from the moment pass.
+  const int K = (mode == 0) ? batch_kick_sums<T>() : 3;
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int g = lane; g < ng; g += 64) {
-    a0 += part[3 * (size_t)g + 0];
-    a1 += part[3 * (size_t)g + 1];
-    a2 += part[3 * (size_t)g + 2];
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      if (q < K) a[q] += part[(size_t)K * g + q];
   }
-  wave_sum3(a0, a1, a2);
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) a[q] += __shfl_xor(a[q], off);
+  }
   if (lane != 0) return;
+  const double ncell = (double)I.nx * I.ny;
+  double m0, m1;
   if (mode == 0) {
-    C.c.sum_rho = a0;
-    C.c.mu[0] = a1 / a0;
-    C.c.mu[1] = a2 / a0;
+    C.c.sum_rho = a[0];
+    C.c.mu[0] = a[1] / a[0];
+    C.c.mu[1] = a[2] / a[0];
+    if (K != 7) return;
+    // sum rho |r - mu|^2 = sum rho |r|^2 - mu . sum rho r; sum m.(r - mu) likewise
+    const double mu0 = C.c.mu[0], mu1 = C.c.mu[1];
+    const double var = a[3] - (mu0 * a[1] + mu1 * a[2]);
+    m0 = 0.5 * C.c.dt * var / ncell;
+    m1 = (a[4] - (mu0 * a[5] + mu1 * a[6])) / ncell;
   } else {
-    const double ncell = (double)I.nx * I.ny;
-    const double m0 = a0 / ncell, m1 = a1 / ncell;
+    m0 = a[0] / ncell;
+    m1 = a[1] / ncell;
+  }
+  {
     const double dedt = C.c.power;
     const double rt = sqrt(m1 * m1 + 4.0 * m0 * dedt);
     double scale = (m1 >= 0.0) ? 2.0 * dedt / (m1 + rt) : (rt - m1) / (2.0 * m0);
@@ -774,7 +833,9 @@ __global__ void __launch_bounds__(256) batch_kick_reduce_kernel(BatchTabs<T> b, 
   }
 }
 
-// Second moments (mode 0) or the kick itself (mode 1) on one tile of a kicking instance.
+// Second moments (mode 0, FP64) or the kick itself (mode 1, forced instances without
+// Smagorinsky) on one tile of a kicking instance.  The moments read the final stage's output
+// wherever it is (kick_reg); the kick works on A in place.
 template <typename T>
 __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl, int mode) {
   const int t = threadIdx.x & 63;
@@ -787,7 +848,7 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
   const int yend = tl.y0 + tl.rows;
   double s0 = 0, s1 = 0, s2 = 0;
   CflMin<T> cfl(I.dx, I.dy, I.cs);
-  gmem_t<T>* A = gmem(I.A);
+  gmem_t<T>* A = gmem(I.A) + (mode == 0 ? (size_t)kick_reg(I) * I.reg_stride : 0);
   const gmem_t<T>* raw = gmem(static_cast<const T*>(I.raw));
   gmem_t<double>* part = gmem(I.part);
   // Rows are walked by every lane (uniform loop); lanes past the strip or the grid skip.
@@ -840,7 +901,7 @@ __device__ void batch_kick_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile&
 
 template <typename T>
 __global__ void __launch_bounds__(kBatchBlock) batch_kick_tiles_kernel(BatchTabs<T> b, int mode) {
-  const int* L = b.lists + WL_KICK * list_stride(b.cap);
+  const int* L = b.lists + (mode == 0 ? WL_KICK : WL_APPLY) * list_stride(b.cap);
   const int total = ((const cmem_t<int>*)L)[1];
   unsigned* ctr = &b.counters[mode == 0 ? CNT_MOM : CNT_APPLY];
   for (int k = wave_first(); k < total; k = wave_then(ctr)) {
@@ -857,8 +918,15 @@ __global__ void __launch_bounds__(kBatchBlock) batch_kick_tiles_kernel(BatchTabs
 // tile's columns, lanes 0 and kTileCols + 1 their neighbours. A cell's primitives are
 // recovered once, as in the single-grid timestep kernel, and the face expressions are the
 // same, so both give the same limits bit for bit.
+//
+// `kick` (a forced Smagorinsky instance ending a step): the rows come from the final stage's
+// output (kick_reg) and each row is kicked as it enters, with batch_kick_tile's expressions
+// (mu, scale and dt from the control block); the lanes that own a cell of the tile's rows
+// write its kicked state to A, and the limits are those of the kicked state.  Halo rows and
+// columns are kicked by every tile that reads them, with the owner's arithmetic.
 template <typename T>
-__device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl) {
+__device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const Tile& tl,
+                                    bool kick) {
   const int t = threadIdx.x & 63;
   const int nx = I.nx, ny = I.ny, nv = I.nv;
   int xc = (tl.x0 - 1 + t) % nx;
@@ -866,45 +934,87 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
   const bool on = t >= 1 && t <= kTileCols && tl.x0 + t - 1 < nx;
   CflMin<T> cfl(I.dx, I.dy, I.cs);
   unsigned long long kmax = 0ull;
-  const gmem_t<T>* A = gmem(static_cast<const T*>(I.A));
+  gmem_t<T>* const A = gmem(I.A);
+  const gmem_t<T>* src = A + (kick ? (size_t)kick_reg(I) * I.reg_stride : 0);
+  const gmem_t<T>* raw = gmem(static_cast<const T*>(I.raw));
   const int j0 = tl.y0, j1 = min(tl.y0 + tl.rows, ny);
-  auto fetch = [&](int j, T (&q)[3]) {
+  T mu0 = T(0), mu1 = T(0), kdt = T(0), scale = T(0);
+  bool off = true;
+  if (kick) {
+    mu0 = T(C.c.mu[0]);
+    mu1 = T(C.c.mu[1]);
+    kdt = T(C.c.dt);
+    off = C.c.power == 0.0;
+    scale = T(C.c.scale);
+  }
+  // A row's raw values at this lane's column: the state, and with the kick the raw force and
+  // the tracer (nvar 4) that the kicked copy carries.
+  struct Row {
+    T q[3];
+    T f[2];
+    T x;
+  };
+  auto fetch = [&](int j, Row& w) {
     if (j < 0 || j >= ny) {
       j %= ny;
       if (j < 0) j += ny;
     }
-    const gmem_t<T>* src = A + ((size_t)j * nx + xc) * nv;
-    q[0] = src[0];
-    q[1] = src[1];
-    q[2] = src[2];
+    const size_t cell = (size_t)j * nx + xc;
+    const gmem_t<T>* q = src + cell * nv;
+    w.q[0] = q[0];
+    w.q[1] = q[1];
+    w.q[2] = q[2];
+    w.f[0] = w.f[1] = w.x = T(0);
+    if (kick) {
+      w.f[0] = raw[cell * 2];
+      w.f[1] = raw[cell * 2 + 1];
+      if (nv > 3) w.x = q[3];
+    }
   };
-  auto prim = [&](const T (&q)[3], T& u, T& v) {
+  // Row j enters: kick it (the owned cells are written to A), then recover.
+  auto prim = [&](int j, Row& w, T& u, T& v) {
+    if (kick) {
+      const T f0 = w.f[0] - mu0, f1 = w.f[1] - mu1;
+      const T acc0 = off ? T(0) : scale * f0;
+      const T acc1 = off ? T(0) : scale * f1;
+      const T rho = w.q[0];
+      const T m0 = w.q[1] + rho * (acc0 * kdt);
+      const T m1 = w.q[2] + rho * (acc1 * kdt);
+      w.q[0] = floor_to(rho, I.dfloor);
+      w.q[1] = m0;
+      w.q[2] = m1;
+      if (on && j >= j0 && j < j1) {
+        gmem_t<T>* d = A + ((size_t)j * nx + xc) * nv;
+        d[0] = w.q[0];
+        d[1] = m0;
+        d[2] = m1;
+        if (nv > 3) d[3] = w.x;
+      }
+    }
     T r;
-    load_prim(q, nv, false, I.dfloor, r, u, v);
+    load_prim(w.q, nv, false, I.dfloor, r, u, v);
   };
   // Rows are fetched two ahead of their use (see the single-grid kernel).
-  T q1[3], q2[3];
+  Row q1, q2;
   fetch(j0, q1);
   fetch(j0 + 1, q2);
   if (!I.smag_on) {
     for (int j = j0; j < j1; ++j) {
       T u, v;
-      prim(q1, u, v);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+      prim(j, q1, u, v);
+      q1 = q2;
       fetch(j + 2, q2);
       if (on) cfl.add(u, v);
     }
   } else {
     T ou_s, ov_s, ou_c, ov_c;
     {
-      T q0[3];
+      Row q0;
       fetch(j0 - 1, q0);
-      prim(q0, ou_s, ov_s);
+      prim(j0 - 1, q0, ou_s, ov_s);
     }
-    prim(q1, ou_c, ov_c);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+    prim(j0, q1, ou_c, ov_c);
+    q1 = q2;
     fetch(j0 + 2, q2);
     T lu_s = __shfl_up(ou_s, 1), lv_s = __shfl_up(ov_s, 1);
     T lu_c = __shfl_up(ou_c, 1), lv_c = __shfl_up(ov_c, 1);
@@ -912,9 +1022,8 @@ __device__ void batch_timestep_tile(const BatchInst<T>& I, BatchCtrl& C, const T
     T ru_c = __shfl_down(ou_c, 1), rv_c = __shfl_down(ov_c, 1);
     for (int j = j0; j < j1; ++j) {
       T ou_n, ov_n;
-      prim(q1, ou_n, ov_n);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q1[k] = q2[k];
+      prim(j + 1, q1, ou_n, ov_n);
+      q1 = q2;
       fetch(j + 3, q2);
       const T lu_n = __shfl_up(ou_n, 1), lv_n = __shfl_up(ov_n, 1);
       const T ru_n = __shfl_down(ou_n, 1), rv_n = __shfl_down(ov_n, 1);
@@ -976,7 +1085,9 @@ __global__ void __launch_bounds__(kBatchBlock) batch_timestep_kernel(BatchTabs<T
     if (only < 0) list_item(b, L, k, s, q);
     const BatchInst<T> I = load_const(b.inst + s);   // by value (see batch_stage_kernel)
     BatchCtrl& C = b.ctrl[s];
-    batch_timestep_tile<T>(I, C, tile_of(I, only >= 0 ? I.rows_cap : C.rows, q));
+    // Within a round (only < 0) the forced Smagorinsky instances are kicked here.
+    batch_timestep_tile<T>(I, C, tile_of(I, only >= 0 ? I.rows_cap : C.rows, q),
+                           only < 0 && I.forced && I.smag_on);
   }
 }
 

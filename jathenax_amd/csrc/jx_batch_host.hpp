@@ -371,9 +371,10 @@ template <typename T> class Batch : public BatchBase {
   };
 
   static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
-  // Kick sums of a forced instance: 3 doubles per (58-column strip, 8-row group).
+  // Kick sums of a forced instance: batch_kick_sums (7 in FP32, 3 in FP64) doubles per
+  // (58-column strip, 8-row group).
   static size_t group_bytes(int nx, int ny) {
-    return (size_t)3 * sizeof(double) * ((nx + kTileCols - 1) / kTileCols) *
+    return (size_t)batch_kick_sums<T>() * sizeof(double) * ((nx + kTileCols - 1) / kTileCols) *
            ((ny + kGroupRows - 1) / kGroupRows);
   }
   static constexpr size_t smem() { return (size_t)4 * kBatchSlice * 64 * sizeof(T); }
@@ -433,18 +434,21 @@ template <typename T> class Batch : public BatchBase {
     return JX_OK;
   }
 
-  // Which optional launches a round needs: the four kick launches while a forced instance
-  // may still step (kKickRound), the Smagorinsky timestep while one with Smagorinsky may
-  // (kSmagRound).  An instance counts until a poll has seen it done (done is final until
-  // the slot is reused).  Without such instances those launches' work lists are empty, so
-  // leaving them out changes nothing but the round's fixed cost.
-  static constexpr int kKickRound = 1, kSmagRound = 2;
+  // Which optional launches a round needs: the kick reduction (FP64: and the second-moment
+  // pass and its reduction) while a forced instance may still step (kKickRound), the kick
+  // pass while a forced instance without Smagorinsky may (kApplyRound; forced Smagorinsky
+  // instances are kicked inside the timestep pass), the Smagorinsky timestep while an
+  // instance with Smagorinsky may (kSmagRound).  An instance counts until a poll has seen it
+  // done (done is final until the slot is reused).  Without such instances those launches'
+  // work lists are empty, so leaving them out changes nothing but the round's fixed cost.
+  static constexpr int kKickRound = 1, kSmagRound = 2, kApplyRound = 4, kRoundKinds = 8;
   int round_kind() const {
     int kind = 0;
     for (int k = 0; k < max_inst_; ++k)
       if (slots_[k].live && !done_seen_[k]) {
         if (h_inst_[k].forced) kind |= kKickRound;
         if (h_inst_[k].smag_on) kind |= kSmagRound;
+        if (h_inst_[k].forced && !h_inst_[k].smag_on) kind |= kApplyRound;
       }
     return kind;
   }
@@ -456,11 +460,16 @@ template <typename T> class Batch : public BatchBase {
     hipLaunchKernelGGL(batch_control_kernel<T>, dim3(1), dim3(kCtrlBlock), 0, stream_, tb);
     hipLaunchKernelGGL(batch_stage_kernel<T>, g, blk, smem(), stream_, tb);
     if (kind & kKickRound) {
+      // FP32: mean force and scale from the final stage's one-pass sums; FP64: mean force,
+      // then the second-moment pass and the scale.
       hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 0);
-      hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 0);
-      hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 1);
-      hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 1);
+      if (batch_kick_sums<T>() != 7) {
+        hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 0);
+        hipLaunchKernelGGL(batch_kick_reduce_kernel<T>, gi, dim3(256), 0, stream_, tb, 1);
+      }
     }
+    if (kind & kApplyRound)
+      hipLaunchKernelGGL(batch_kick_tiles_kernel<T>, g, blk, 0, stream_, tb, 1);
     if (kind & kSmagRound)
       hipLaunchKernelGGL(batch_timestep_kernel<T>, g, blk, 0, stream_, tb, -1);
   }
@@ -479,7 +488,7 @@ template <typename T> class Batch : public BatchBase {
   }
 
   void drop_graph() {
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kRoundKinds; ++k) {
       if (graph_exec_[k]) hipGraphExecDestroy(graph_exec_[k]);
       if (graph_[k]) hipGraphDestroy(graph_[k]);
       graph_exec_[k] = nullptr;
@@ -514,8 +523,8 @@ template <typename T> class Batch : public BatchBase {
   std::map<size_t, size_t> free_;
   bool use_graph_ = true;
   int graph_rounds_ = 6;
-  hipGraph_t graph_[4] = {};
-  hipGraphExec_t graph_exec_[4] = {};
+  hipGraph_t graph_[kRoundKinds] = {};
+  hipGraphExec_t graph_exec_[kRoundKinds] = {};
 };
 
 }  // namespace jx
