@@ -45,9 +45,23 @@ def _specs():
         U = U + 0.01 * rng.standard_normal(U.shape) * (np.arange(U.shape[-1]) > 0)
         out.append(dict(U=U, dx=1 / nx, dy=1 / ny, integ=integ, kw=dict(reconstruct=recon,
                                                                         rsolver=rs, **phys)))
+    # forced instances of every kick arrangement: with Smagorinsky the final stage writes a
+    # scratch register (RK3: B, with a tracer the kick carries; RK4: C, beside S) and the
+    # timestep pass kicks it into A; without Smagorinsky the kick is a pass of its own
+    modes = jf.cartesian_modes(1, 3, (-3, 3), (-3, 3))
+    for (nx, ny), recon, rs, integ, nv, phys, seed in (
+            ((66, 52), "ppm4", "roe", "rk3", 4, dict(viscosity=1e-3, smagorinsky_cs=0.17), 11),
+            ((60, 70), "plm", "hlle", "rk4", 3, dict(smagorinsky_cs=0.2, drag_rate=0.3), 12),
+            ((64, 64), "ppm4", "llf", "rk3", 3, dict(viscosity=1e-3), 13)):
+        coef = jf.modal_coefficients(modes, (1.0, 1.0), 1, 3, 2.0, 1.0, seed, 0.75, 0.002, 40)
+        basis = jf.forcing_basis(modes, problems.cell_centres(nx), problems.cell_centres(ny),
+                                 (1.0, 1.0))
+        U = problems.kelvin_helmholtz_reference(nx, ny) if nv == 4 else problems.at_rest(nx, ny)
+        out.append(dict(U=U, dx=1 / nx, dy=1 / ny, integ=integ,
+                        kw=dict(reconstruct=recon, rsolver=rs, forcing=(coef, basis),
+                                dt_update=0.002, tdriv_start=0.0, dedt=0.5, **phys)))
     # forced instance
     nx, ny = 72, 60
-    modes = jf.cartesian_modes(1, 3, (-3, 3), (-3, 3))
     coef = jf.modal_coefficients(modes, (1.0, 1.0), 1, 3, 2.0, 1.0, 1618, 0.75, 0.002, 40)
     basis = jf.forcing_basis(modes, problems.cell_centres(nx), problems.cell_centres(ny),
                              (1.0, 1.0))
