@@ -319,18 +319,25 @@ def test_fp32_driven_run_tracks_fp64(jx, viscosity):
     assert np.max(np.abs(U32 - U64)) <= 2e-5 * np.max(np.abs(U64))
 
 
-@pytest.mark.parametrize("shape,nlim", [((48, 40), 24), ((300, 173), 6)])
-def test_fp32_evaluated_forcing_equals_rendered(jx, shape, nlim):
+@pytest.mark.parametrize("shape,nlim,tracer", [((48, 40), 24, False), ((300, 173), 6, False),
+                                               ((66, 52), 12, True)])
+def test_fp32_evaluated_forcing_equals_rendered(jx, shape, nlim, tracer):
     """FP32 forced Smagorinsky runs (the driven-turbulence configuration) evaluate the raw
     force where the kick reads it, from per-row sums of the basis's mode groups; rendering
-    the (ny, nx, 2) array every step instead must give the same run bit for bit."""
+    the (ny, nx, 2) array every step instead must give the same run bit for bit (with a
+    tracer, which the kick carries unchanged)."""
     from jathenax_amd import workloads
     from jathenax_amd.device import context_for
     nx, ny = shape
     s = workloads.turbulence_spec(nx, ny, 0.5, 1618, 0.002, "rk2", "wenoz", "hlle", cfl=0.3,
                                   dtype=np.float32)
     s["kw"]["dedt"] = 0.5
-    U0 = torch.tensor(s["U"], dtype=torch.float32, device="cuda")
+    U = s["U"]
+    if tracer:
+        ys = (np.arange(ny) + 0.5) / ny
+        conc = (np.abs(ys - 0.5) < 0.25).astype(np.float64)[:, None] * np.ones((1, nx))
+        U = np.concatenate([U, (U[..., 0] * conc)[..., None]], axis=-1)
+    U0 = torch.tensor(U, dtype=torch.float32, device="cuda")
     args = (s["dx"], s["dy"], s["cs"], s["dfloor"], s["integrator"], s["cfl"], s["tlim"], nlim)
     ctx = context_for(U0)
     out = {}
