@@ -280,6 +280,13 @@ int jx_batch_launch(jx_batch* b, int rounds);
 int jx_batch_wait(jx_batch* b, int* active_out);
 /* Per slot: done (1), running (0) or empty (-1), time and cycle. */
 int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap);
+/* The same, and the active count, as of the end of the oldest jx_batch_launch not yet
+ * polled (every launch snapshots the loop state; two are kept, a third drops the oldest;
+ * jx_batch_wait, jx_batch_run and jx_batch_poll clear them).  Waits for that launch only,
+ * so a caller that keeps the next launch queued never leaves the GPU idle while it reads.
+ * No counterpart in the reference (batching is new). */
+int jx_batch_poll_launched(jx_batch* b, int* done, double* time, int* cycle, int cap,
+                           int* active_out);
 /* Copy an instance's state to U_out (may be NULL) and return (time, last dt, cycle). */
 int jx_batch_result(jx_batch* b, int id, void* U_out, double* time, double* dt, int* cycle);
 int jx_batch_state(jx_batch* b, int id, void** state_out);

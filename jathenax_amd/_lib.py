@@ -119,6 +119,8 @@ SIGNATURES = {
     "jx_batch_launch": (c_int, [c_void_p, c_int]),
     "jx_batch_wait": (c_int, [c_void_p, POINTER(c_int)]),
     "jx_batch_poll": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double), POINTER(c_int), c_int]),
+    "jx_batch_poll_launched": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double),
+                                       POINTER(c_int), c_int, POINTER(c_int)]),
     "jx_batch_result": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_double), POINTER(c_double),
                                 POINTER(c_int)]),
     "jx_batch_state": (c_int, [c_void_p, c_int, POINTER(c_void_p)]),

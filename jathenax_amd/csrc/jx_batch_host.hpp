@@ -26,6 +26,7 @@ struct BatchBase {
   virtual int launch(int rounds) = 0;
   virtual int wait(int* active) = 0;
   virtual int poll(int* done, double* time, int* cycle, int cap) = 0;
+  virtual int poll_launched(int* done, double* time, int* cycle, int cap, int* active) = 0;
   virtual int result(int id, void* U_out, double* time, double* dt, int* cycle) = 0;
   virtual int state_ptr(int id, void** p) = 0;
   virtual int set_graph(int on, int rounds) = 0;
@@ -72,6 +73,12 @@ template <typename T> class Batch : public BatchBase {
             hipSuccess ||
         hipHostMalloc(&h_meta_, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess)
       return fail("batch allocation failed (pool too large for the device?)");
+    for (int k = 0; k < 2; ++k)
+      if (hipHostMalloc(&h_snap_[k], sizeof(BatchCtrl) * max_inst_, hipHostMallocDefault) !=
+              hipSuccess ||
+          hipHostMalloc(&h_snap_meta_[k], sizeof(int) * 8, hipHostMallocDefault) != hipSuccess ||
+          hipEventCreateWithFlags(&snap_ev_[k], hipEventDisableTiming) != hipSuccess)
+        return fail("batch snapshot allocation failed");
     hipMemsetAsync(d_ctrl_, 0, sizeof(BatchCtrl) * max_inst_, stream_);
     hipMemsetAsync(d_meta_, 0, sizeof(int) * 8, stream_);
     hipMemsetAsync(d_cnt_, 0, sizeof(unsigned) * 8, stream_);
@@ -94,6 +101,11 @@ template <typename T> class Batch : public BatchBase {
       if (p) hipFree(p);
     if (h_ctrl_) hipHostFree(h_ctrl_);
     if (h_meta_) hipHostFree(h_meta_);
+    for (int k = 0; k < 2; ++k) {
+      if (h_snap_[k]) hipHostFree(h_snap_[k]);
+      if (h_snap_meta_[k]) hipHostFree(h_snap_meta_[k]);
+      if (snap_ev_[k]) hipEventDestroy(snap_ev_[k]);
+    }
     if (stream_) hipStreamDestroy(stream_);
   }
 
@@ -276,6 +288,21 @@ template <typename T> class Batch : public BatchBase {
       for (int k = 0; k < rounds; ++k) launch_round(kind);
     }
     if (hipGetLastError() != hipSuccess) return fail("round launch failed");
+    // Snapshot of the loop state as of the end of these rounds (poll_launched).  With two
+    // launches already unpolled, the older one's snapshot is dropped.
+    if (snap_count_ == 2) {
+      snap_tail_ ^= 1;
+      snap_count_ = 1;
+    }
+    const int k = snap_head_;
+    if (hipMemcpyAsync(h_snap_[k], d_ctrl_, sizeof(BatchCtrl) * max_inst_,
+                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipMemcpyAsync(h_snap_meta_[k], d_meta_, sizeof(int) * 8, hipMemcpyDeviceToHost,
+                       stream_) != hipSuccess ||
+        hipEventRecord(snap_ev_[k], stream_) != hipSuccess)
+      return fail("launch: snapshot failed");
+    snap_head_ ^= 1;
+    ++snap_count_;
     return JX_OK;
   }
 
@@ -284,6 +311,7 @@ template <typename T> class Batch : public BatchBase {
             hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
       return fail("run: synchronisation failed");
+    snap_count_ = 0;   // everything launched has been waited for
     if (active) *active = h_meta_[2];
     return JX_OK;
   }
@@ -293,14 +321,33 @@ template <typename T> class Batch : public BatchBase {
                        stream_) != hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
       return fail("poll failed");
+    snap_count_ = 0;
+    read_ctrl(h_ctrl_, done, time, cycle, cap);
+    return JX_OK;
+  }
+
+  // The loop state as of the end of the oldest launch not yet polled: waits for that
+  // launch's snapshot only, so later launches keep the GPU busy meanwhile.
+  int poll_launched(int* done, double* time, int* cycle, int cap, int* active) override {
+    if (!snap_count_) return fail("poll_launched: no launch is waiting to be polled");
+    const int k = snap_tail_;
+    if (hipEventSynchronize(snap_ev_[k]) != hipSuccess)
+      return fail("poll_launched: synchronisation failed");
+    read_ctrl(h_snap_[k], done, time, cycle, cap);
+    if (active) *active = h_snap_meta_[k][2];
+    snap_tail_ ^= 1;
+    --snap_count_;
+    return JX_OK;
+  }
+
+  void read_ctrl(const BatchCtrl* h, int* done, double* time, int* cycle, int cap) {
     for (int k = 0; k < std::min(cap, max_inst_); ++k) {
       const bool live = slots_[k].live;
-      if (live) done_seen_[k] = h_ctrl_[k].c.done ? 1 : 0;
-      if (done) done[k] = live ? (h_ctrl_[k].c.done ? 1 : 0) : -1;
-      if (time) time[k] = h_ctrl_[k].c.time;
-      if (cycle) cycle[k] = h_ctrl_[k].c.cycle;
+      if (live) done_seen_[k] = h[k].c.done ? 1 : 0;
+      if (done) done[k] = live ? (h[k].c.done ? 1 : 0) : -1;
+      if (time) time[k] = h[k].c.time;
+      if (cycle) cycle[k] = h[k].c.cycle;
     }
-    return JX_OK;
   }
 
   int result(int id, void* U_out, double* time, double* dt, int* cycle) override {
@@ -517,6 +564,12 @@ template <typename T> class Batch : public BatchBase {
   int* d_gscratch_ = nullptr;
   BatchCtrl* h_ctrl_ = nullptr;
   int* h_meta_ = nullptr;
+  // Pinned snapshots of the control table and counters taken at the end of each launch,
+  // two deep, with their events (poll_launched reads the oldest unpolled one).
+  BatchCtrl* h_snap_[2] = {};
+  int* h_snap_meta_[2] = {};
+  hipEvent_t snap_ev_[2] = {};
+  int snap_head_ = 0, snap_tail_ = 0, snap_count_ = 0;
   std::vector<BatchInst<T>> h_inst_;
   std::vector<Slot> slots_;
   std::vector<char> done_seen_;   // per slot: a poll has seen the instance done

@@ -1467,6 +1467,11 @@ int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap) {
   return b ? b->impl->poll(done, time, cycle, cap) : JX_EINVAL;
 }
 
+int jx_batch_poll_launched(jx_batch* b, int* done, double* time, int* cycle, int cap,
+                           int* active_out) {
+  return b ? b->impl->poll_launched(done, time, cycle, cap, active_out) : JX_EINVAL;
+}
+
 int jx_batch_result(jx_batch* b, int id, void* U_out, double* time, double* dt, int* cycle) {
   return b ? b->impl->result(id, U_out, time, dt, cycle) : JX_EINVAL;
 }

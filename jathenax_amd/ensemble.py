@@ -170,6 +170,17 @@ class Ensemble:
                 return done
         return done
 
+    def poll_launched(self):
+        """(done, time, cycle, active) as of the end of the oldest launch() not yet polled:
+        waits for that launch only, so a caller that keeps the next launch queued leaves the
+        GPU no idle time while it reads the flags."""
+        n = self.max_instances
+        d, t, c = (ctypes.c_int * n)(), (ctypes.c_double * n)(), (ctypes.c_int * n)()
+        act = ctypes.c_int()
+        self._check(_lib.load().jx_batch_poll_launched(self.handle, d, t, c, n,
+                                                       ctypes.byref(act)), "poll_launched")
+        return np.asarray(d[:]), np.asarray(t[:]), np.asarray(c[:]), act.value
+
     def poll(self):
         n = self.max_instances
         d, t, c = (ctypes.c_int * n)(), (ctypes.c_double * n)(), (ctypes.c_int * n)()

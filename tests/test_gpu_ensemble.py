@@ -201,3 +201,45 @@ def test_results_do_not_depend_on_tile_height(jx):
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(_np(a[0]), _np(b[0]))
             assert a[1:] == b[1:]
+
+
+def test_pipelined_polling(jx):
+    """Every launch snapshots the loop state at its end; poll_launched returns the snapshots
+    in launch order while later launches run.  They equal a synchronous run's state after the
+    same rounds, and the results of a pipelined run equal a synchronous run's bit for bit."""
+    from jathenax_amd.ensemble import Ensemble
+    specs = _specs()[:4] + _specs()[-2:]
+
+    def make():
+        ens = Ensemble(dtype=torch.float32, max_instances=8, pool_bytes=1 << 27)
+        ids = [ens.add(torch.tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"],
+                       s["dy"], 1.0, 1e-10, s["integ"], 0.4, 0.04, 10 ** 6, **s["kw"])
+               for s in specs]
+        return ens, ids
+
+    a, ida = make()
+    a.run(6)
+    _, t6, c6 = a.poll()
+    a.run(6)
+    _, t12, c12 = a.poll()
+    b, idb = make()
+    b.launch(6)
+    b.launch(6)
+    _, s6, k6, _ = b.poll_launched()
+    _, s12, k12, _ = b.poll_launched()
+    assert list(k6[idb]) == list(c6[ida]) and list(k12[idb]) == list(c12[ida])
+    assert list(s6[idb]) == list(t6[ida]) and list(s12[idb]) == list(t12[ida])
+    with pytest.raises(ValueError):
+        b.poll_launched()
+    a.run_until_done(chunk=6)
+    b.launch(6)
+    while True:
+        b.launch(6)
+        done, _, _, active = b.poll_launched()
+        if active == 0:
+            break
+    b.wait()
+    for i, j in zip(ida, idb):
+        ra, rb = a.result(i), b.result(j)
+        np.testing.assert_array_equal(_np(ra[0]), _np(rb[0]))
+        assert ra[1:] == rb[1:]
