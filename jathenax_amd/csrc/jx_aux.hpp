@@ -526,16 +526,32 @@ __device__ __forceinline__ float4 group_row_sums(const RenderArgs<float>& a, con
   const int* sign = order + M;
   const float nan = __builtin_nanf("");
   float p0 = 0.0f, q0 = 0.0f, p1 = 0.0f, q1 = 0.0f;
-  for (int k = gstart[g]; k < gstart[g + 1]; ++k) {
-    const int m = order[k];
-    const float s = (float)sign[m];
-    const float cyv = a.cy[(size_t)m * ny + j], syv = a.sy[(size_t)m * ny + j];
-    const float a00 = fill ? nan : amp[m * 4 + 0], a01 = fill ? nan : amp[m * 4 + 1];
-    const float a10 = fill ? nan : amp[m * 4 + 2], a11 = fill ? nan : amp[m * 4 + 3];
-    p0 = fmaf(cyv, a00, fmaf(-syv, a01, p0));
-    q0 = fmaf(s, fmaf(syv, a00, cyv * a01), q0);
-    p1 = fmaf(cyv, a10, fmaf(-syv, a11, p1));
-    q1 = fmaf(s, fmaf(syv, a10, cyv * a11), q1);
+  // Members in chunks of 8: the chunk's basis values are loaded first, then accumulated in
+  // member order (the order fixes the sums' rounding).
+  constexpr int C = 8;
+  const int k0 = gstart[g], k1 = gstart[g + 1];
+  for (int kc = k0; kc < k1; kc += C) {
+    int mm[C];
+    float cyv[C], syv[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      mm[u] = (kc + u < k1) ? order[kc + u] : order[kc];
+      cyv[u] = a.cy[(size_t)mm[u] * ny + j];
+      syv[u] = a.sy[(size_t)mm[u] * ny + j];
+    }
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      if (kc + u < k1) {
+        const int m = mm[u];
+        const float s = (float)sign[m];
+        const float a00 = fill ? nan : amp[m * 4 + 0], a01 = fill ? nan : amp[m * 4 + 1];
+        const float a10 = fill ? nan : amp[m * 4 + 2], a11 = fill ? nan : amp[m * 4 + 3];
+        p0 = fmaf(cyv[u], a00, fmaf(-syv[u], a01, p0));
+        q0 = fmaf(s, fmaf(syv[u], a00, cyv[u] * a01), q0);
+        p1 = fmaf(cyv[u], a10, fmaf(-syv[u], a11, p1));
+        q1 = fmaf(s, fmaf(syv[u], a10, cyv[u] * a11), q1);
+      }
+    }
   }
   return make_float4(p0, q0, p1, q1);
 }
@@ -719,11 +735,14 @@ __global__ void __launch_bounds__(kAuxBlock) kick_first_moments_kernel(KickArgs<
 // FP32 single pass, 7 sums per block: rho, rho r, rho |r|^2, m.r, m): mu, then
 //   sum rho |r - mu|^2 = sum rho |r|^2 - mu . sum rho r,   sum m.(r - mu) = sum m.r - mu . sum m,
 // and the scale from them.
-static __global__ void reduce_kick_kernel(Ctrl* c, const double* partial, int nblocks, int mode,
-                                   double ncells) {
+// One block of kReduceBlock threads: each thread sums a strided share of the partials in a
+// fixed order, then a fixed tree in LDS.
+constexpr int kReduceBlock = 1024;
+static __global__ void __launch_bounds__(kReduceBlock)
+reduce_kick_kernel(Ctrl* c, const double* partial, int nblocks, int mode, double ncells) {
   if (c->done) return;
   constexpr int K = 7;
-  __shared__ double s[K][256];
+  __shared__ double s[K][kReduceBlock];
   const int nk = mode == 2 ? 7 : 3;
   double a[K];
 #pragma unroll

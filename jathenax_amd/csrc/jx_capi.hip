@@ -529,7 +529,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
     // Kick: forcing.py:137-149 then hydro.py:132, on the post-drag state in A.
     const size_t n = (size_t)p->nx * p->ny;
     const bool one_pass = s.visc && jx::kick_sums<T, true>() == 7;   // as the stage kernel
-    hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+    hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(jx::kReduceBlock), 0, c->stream, c->ctrl,
                        w.partial, (int)nblocks, one_pass ? 2 : 0, (double)n);
     jx::KickArgs<T> ka{};
     ka.u = fused_kick ? const_cast<T*>(kick_src) : A;   // the moment pass reads it only
@@ -547,7 +547,7 @@ int launch_step(jx_ctx* c, const StepPlan<T>& s, const StepScratch<T>& w, const 
     if (!one_pass) {
       hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0,
                          c->stream, ka);
-      hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+      hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(jx::kReduceBlock), 0, c->stream, c->ctrl,
                          w.partial, (int)g.x, 1, (double)n);
     }
     if (fused_kick) {
@@ -815,10 +815,10 @@ int normalize_impl(jx_ctx* c, int ny, int nx, int nvar, const T* U, const T* raw
   ka.ctrl = c->ctrl;
   ka.partial = c->partial;
   hipLaunchKernelGGL(jx::kick_first_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
-  hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+  hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(jx::kReduceBlock), 0, c->stream, c->ctrl,
                      c->partial, (int)g.x, 0, (double)n);
   hipLaunchKernelGGL(jx::kick_second_moments_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream, ka);
-  hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(256), 0, c->stream, c->ctrl,
+  hipLaunchKernelGGL(jx::reduce_kick_kernel, dim3(1), dim3(jx::kReduceBlock), 0, c->stream, c->ctrl,
                      c->partial, (int)g.x, 1, (double)n);
   hipLaunchKernelGGL(jx::kick_acceleration_kernel<T>, g, dim3(jx::kAuxBlock), 0, c->stream,
                      (const Ctrl*)c->ctrl, raw, acc, n);
