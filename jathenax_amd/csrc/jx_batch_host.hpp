@@ -370,7 +370,8 @@ template <typename T> class Batch : public BatchBase {
   // State and loop state between rounds.  An instance whose final stage has run but whose
   // step the next control round would commit is reported committed (time + dt, cycle + 1);
   // an instance in mid-step is reported at the start of that step, whose state A still
-  // holds (the stage registers only write A in the final stage).
+  // holds (only the round of a step's final stage writes A: the final stage itself or, for
+  // a forced Smagorinsky instance, the timestep pass that kicks its output).
   int checkpoint(int id, void* U_out, jx_loop_state* st) override {
     if (!st) return fail("checkpoint: null state");
     double t = 0.0, dt = 0.0;
