@@ -131,3 +131,23 @@ def test_serving_workload_alone_equals_in_mix():
         assert alone.status == "done" and alone.path == r.path
         np.testing.assert_array_equal(alone.result[0].cpu().numpy(), r.result[0].cpu().numpy())
         assert alone.result[1:] == r.result[1:]
+
+
+def test_headline_fp32_accuracy_at_t1(jx):
+    """The FP32 accuracy gate of the 4096^2 smooth-wave workload (PPM4 + Roe, RK3, CFL 0.4):
+    density L1 against the FP64 form at t = 1 at most 1e-5 (about 11770 steps; measured
+    1.5e-6), both runs ending at t = 1 exactly (their dt sequences may differ in the last
+    bits, so the cycle counts may differ by one)."""
+    from jathenax_amd import problems
+    n = 4096
+    U0 = problems.smooth_wave(n, n)
+    args = (1 / n, 1 / n, 1.0, 1e-10, "rk3", 0.4, 1.0, 10 ** 6)
+    U64, t64, _, c64 = jx.evolve_isothermal(torch.tensor(U0, device="cuda"), *args)
+    rho64 = U64[..., 0].clone()
+    del U64
+    U32, t32, _, c32 = jx.evolve_isothermal(torch.tensor(U0, dtype=torch.float32,
+                                                         device="cuda"), *args)
+    assert t64 == t32 == 1.0 and abs(c32 - c64) <= 1
+    l1 = float((U32[..., 0].double() - rho64).abs().mean())
+    print(f"4096^2 smooth wave t=1: FP32 vs FP64 L1(rho) {l1:.3e}, cycles {c32} / {c64}")
+    assert l1 <= 1e-5
