@@ -67,6 +67,15 @@ def build(jobs=None, verbose=False, only=None):
         missing = [obj for src, obj, _ in cmds if src.stem not in only and not obj.exists()]
         if missing:
             raise RuntimeError(f"no earlier object for {missing[0].name}: run a full build")
+        # Every unit includes the shared headers (kernel argument structs among them), so an
+        # object older than any header may disagree with the rebuilt units on a layout.
+        headers = [*CSRC.glob("*.hpp"), *(PKG.parent / "include").glob("*.h")]
+        newest = max((h.stat().st_mtime for h in headers), default=0.0)
+        stale = [obj for src, obj, _ in cmds
+                 if src.stem not in only and obj.stat().st_mtime < newest]
+        if stale:
+            raise RuntimeError(f"{stale[0].name} is older than a shared header: run a full "
+                               "build")
         keep = [(src, obj, None) for src, obj, _ in cmds if src.stem not in only]
         cmds = [c for c in cmds if c[0].stem in only]
 

@@ -311,7 +311,7 @@ template <typename T> class Batch : public BatchBase {
             hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
       return fail("run: synchronisation failed");
-    snap_count_ = 0;   // everything launched has been waited for
+    drop_snapshots();   // everything launched has been waited for
     if (active) *active = h_meta_[2];
     return JX_OK;
   }
@@ -321,9 +321,15 @@ template <typename T> class Batch : public BatchBase {
                        stream_) != hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
       return fail("poll failed");
-    snap_count_ = 0;
+    drop_snapshots();
     read_ctrl(h_ctrl_, done, time, cycle, cap);
     return JX_OK;
+  }
+
+  // Empties the snapshot ring: the next launch's snapshot is the next one polled.
+  void drop_snapshots() {
+    snap_tail_ = snap_head_;
+    snap_count_ = 0;
   }
 
   // The loop state as of the end of the oldest launch not yet polled: waits for that

@@ -231,6 +231,16 @@ def test_pipelined_polling(jx):
     assert list(s6[idb]) == list(t6[ida]) and list(s12[idb]) == list(t12[ida])
     with pytest.raises(ValueError):
         b.poll_launched()
+    # wait() settles every launch, so the snapshot polled next is the next launch's, also
+    # when the launch before the wait was never polled.
+    b.launch(6)
+    b.wait()
+    b.launch(6)
+    _, s24, k24, _ = b.poll_launched()
+    a.run(6)
+    a.run(6)
+    _, t24, c24 = a.poll()
+    assert list(k24[idb]) == list(c24[ida]) and list(s24[idb]) == list(t24[ida])
     a.run_until_done(chunk=6)
     b.launch(6)
     while True:
