@@ -11,7 +11,9 @@ seconds and through size-independent properties where it does not.
   benchmark uses against the oracle (<= 1e-13), and FP32 against FP64 after five steps;
 * serving: requests drawn by the serving workload generator at its own size and end-time
   ranges (square grids 128..2048, tlim U[0.1, 2]), each result served in the mix equal bit
-  for bit to the same request served alone.
+  for bit to the same request served alone;
+* a 16384 x 12000 grid in FP32 and FP64: translation invariance bit for bit, and mass
+  conservation.
 """
 
 import numpy as np
@@ -151,3 +153,33 @@ def test_headline_fp32_accuracy_at_t1(jx):
     l1 = float((U32[..., 0].double() - rho64).abs().mean())
     print(f"4096^2 smooth wave t=1: FP32 vs FP64 L1(rho) {l1:.3e}, cycles {c32} / {c64}")
     assert l1 <= 1e-5
+
+
+@pytest.mark.parametrize("dtype,steps", [(torch.float32, 3), (torch.float64, 1)])
+def test_translation_invariance_on_a_large_grid(jx, dtype, steps):
+    """16384 x 12000 (ragged strips and bands, 2.4 / 4.7 GB per state): on a periodic grid
+    the update of every cell depends only on its neighbourhood and the grid-wide dt limit, so
+    evolving a state shifted by (4999, 7777) cells gives the shifted result bit for bit.
+    Any index that strays from its cell's neighbourhood at this size breaks the equality."""
+    nx, ny = 16384, 12000
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = (torch.arange(nx, device="cuda", dtype=torch.float64) + 0.5) / nx
+    y = (torch.arange(ny, device="cuda", dtype=torch.float64) + 0.5) / ny
+    X, Y = x[None, :], y[:, None]
+    rho = 1.0 + 0.2 * torch.sin(2 * np.pi * X) * torch.cos(4 * np.pi * Y)
+    rho = rho + 0.01 * torch.rand((ny, nx), generator=g, device="cuda", dtype=torch.float64)
+    vx = 0.3 * torch.sin(2 * np.pi * Y + 1.0).expand(ny, nx)
+    vy = 0.2 * torch.cos(2 * np.pi * X).expand(ny, nx)
+    U = torch.stack((rho, rho * vx, rho * vy), dim=-1).to(dtype).contiguous()
+    del rho, vx, vy
+    shift = (4999, 7777)
+    Ur = torch.roll(U, shifts=shift, dims=(0, 1)).contiguous()
+    args = (1 / nx, 1 / ny, 1.0, 1e-10, "rk3", 0.4, 10.0, steps)
+    print(f"{nx}x{ny} {dtype} translation invariance, {steps} steps", flush=True)
+    out, t, dt, cyc = jx.evolve_isothermal(U, *args)
+    outr, tr, dtr, cycr = jx.evolve_isothermal(Ur, *args)
+    assert cyc == cycr == steps and t == tr and dt == dtr
+    assert torch.equal(torch.roll(out, shifts=shift, dims=(0, 1)), outr)
+    m0 = float(U[..., 0].double().sum())
+    m1 = float(out[..., 0].double().sum())
+    assert abs(m1 - m0) <= (1e-6 if dtype == torch.float32 else 1e-12) * abs(m0)
