@@ -15,10 +15,15 @@ removed and requeued; resumed later, it continues bit for bit, so pre-emption ch
 latency.
 
 Large requests: a request of at least `big_zones` cells runs on the single-grid fused
-kernel (the device-resident loop, `big_chunk` steps per server iteration) on a stream of its
-own, while the batch's rounds run on the engine's stream: one grid of that size fills the
-GPU better alone than as tiles among others. At most `max_big` run at once; the rest wait
-without blocking smaller requests. A request's path depends only on its size, so its result
+kernel (the device-resident loop, in chunks of `big_chunk` steps) on a stream of its own,
+while the batch's rounds run on the engine's stream: one grid of that size fills the GPU
+better alone than as tiles among others. By default each large request runs in a worker
+thread (the library's calls release the interpreter lock), so the server loop keeps
+launching and retiring batch rounds while the large request's chunks run, and two large
+requests run at once (a grid of 1024^2 to 1448^2 cells leaves much of the GPU idle); with
+`big_threads=False` the server loop runs one chunk of every large request per iteration
+itself, one large request at a time. At most `max_big` run at once; the rest wait without
+blocking smaller requests. A request's path depends only on its size, so its result
 is the same alone as in any mix. In FP32 the two paths agree to round-off (the kernels are
 separate code that may fuse different products into adds, and forced runs reduce their kick
 sums in different orders).
@@ -31,6 +36,8 @@ fixed tiles), so a result served in a mix equals the same request served alone.
 """
 
 import math
+import queue as _queue
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -69,7 +76,8 @@ class Request:
 class Server:
     def __init__(self, pool_bytes=None, max_instances=256, dtype=torch.float32,
                  rounds_per_poll=6, keep_results=True, graph=True, preempt=True,
-                 big_zones=1 << 20, big_chunk=16, max_big=1, big_rounds=4):
+                 big_zones=1 << 20, big_chunk=None, max_big=None, big_rounds=4,
+                 big_threads=True):
         self.ens = Ensemble(dtype=dtype, max_instances=max_instances, pool_bytes=pool_bytes,
                             graph=graph, graph_rounds=rounds_per_poll)
         self.dtype = dtype
@@ -77,15 +85,28 @@ class Server:
         self.keep_results = keep_results
         self.preempt = preempt
         self.big_zones = big_zones          # 0: every request takes the batch path
-        self.big_chunk = big_chunk
-        self.max_big = max_big
-        # While large requests run, each server iteration launches big_rounds times as many
-        # batch rounds, so the batch keeps the GPU busy during the large requests' chunks.
+        self.big_threads = big_threads
+        # Steps per chunk of a large request: a worker thread checks the request (finished,
+        # non-finite, over budget, server stopping) between chunks; the server loop's own
+        # chunks are shorter, since the loop waits for them.
+        self.big_chunk = big_chunk or (256 if big_threads else 16)
+        self.max_big = max_big or (2 if big_threads else 1)
+        # Without worker threads, each server iteration launches big_rounds times as many
+        # batch rounds while large requests run, so the batch keeps the GPU busy during the
+        # large requests' chunks.
         self.big_rounds = big_rounds
         self.queue = deque()
         self.running = {}
         self.grid_running = {}
         self.grid_stream = torch.cuda.Stream()
+        # Worker threads: one stream per slot (each stream has its own library context and
+        # scratch, reused by the slot's later requests); finished requests come back through
+        # grid_done.
+        self.grid_streams = ([torch.cuda.Stream() for _ in range(self.max_big)]
+                             if big_threads else [])
+        self.grid_done = _queue.SimpleQueue()
+        self.workers = {}
+        self.stop = threading.Event()
         self.finished = []
         self.clock0 = None
 
@@ -146,6 +167,19 @@ class Server:
             req.admitted = self.now()
         req.status = "running"
         self.grid_running[req.rid] = req
+        if self.big_threads:
+            self._start_worker(req)
+
+    def _start_worker(self, req):
+        # The state was made on this thread's stream; the worker's stream waits for it.
+        ready = torch.cuda.Event()
+        ready.record()
+        busy = {w[1] for w in self.workers.values()}
+        slot = next(k for k in range(self.max_big) if k not in busy)
+        th = threading.Thread(target=self._grid_worker, args=(req, slot, ready),
+                              name=f"grid-{req.rid}", daemon=True)
+        self.workers[req.rid] = (th, slot)
+        th.start()
 
     def _admit(self):
         while True:
@@ -203,6 +237,65 @@ class Server:
                 del self.running[iid]
                 self.finished.append(req)
 
+    def _grid_finish(self, req, U, st):
+        """True (and the request closed) when a large request is done, non-finite or over
+        its budget."""
+        s = req.spec
+        bad = not math.isfinite(st.time)
+        if not bad and st.time < s["tlim"] and st.cycle < s["nlim"] \
+                and req.rounds <= req.max_rounds:
+            return False
+        req.finished = self.now()
+        req.info["cycles"] = int(st.cycle)
+        if bad or not torch.isfinite(U).all():
+            req.status, req.info["error"] = "error", "non-finite state"
+        elif st.time < s["tlim"] and st.cycle < s["nlim"]:
+            req.status, req.info["error"] = "error", "round budget exceeded"
+        else:
+            req.status = "done"
+        req.result = (U, st.time, st.dt, st.cycle) if self.keep_results else None
+        req.state = None
+        return True
+
+    def _grid_worker(self, req, slot, ready):
+        """A large request in chunks of big_chunk steps on the slot's stream, until it is
+        closed or the server stops (the request then keeps its state and stays running)."""
+        from .run import evolve_resumable
+        stream = self.grid_streams[slot]
+        try:
+            with torch.cuda.stream(stream):
+                stream.wait_event(ready)
+                s = req.spec
+                U, st = req.state
+                while not self.stop.is_set():
+                    target = min(s["nlim"], (st.cycle if st else 0) + self.big_chunk)
+                    U, st = evolve_resumable(U, s["dx"], s["dy"], s["cs"], s["dfloor"],
+                                             s["integrator"], s["cfl"], s["tlim"], target,
+                                             state=st, **s["kw"])
+                    req.state = (U, st)
+                    req.rounds += self.big_chunk
+                    if self._grid_finish(req, U, st):
+                        break
+                stream.synchronize()
+        except Exception as e:   # reported as the request's error, not the server's
+            req.finished = self.now()
+            req.status, req.info["error"] = "error", f"{type(e).__name__}: {e}"
+            req.state = None
+        self.grid_done.put(req)
+
+    def _collect_grid(self):
+        """Close the large requests whose worker has ended."""
+        while True:
+            try:
+                req = self.grid_done.get_nowait()
+            except _queue.Empty:
+                return
+            th, _ = self.workers.pop(req.rid)
+            th.join()
+            if req.status != "running":
+                del self.grid_running[req.rid]
+                self.finished.append(req)
+
     def _step_grid(self):
         """One chunk of steps for every large request, on the grid stream (the batch's
         launched rounds run meanwhile on the engine's stream)."""
@@ -219,19 +312,7 @@ class Server:
                                          **s["kw"])
                 req.state = (U, st)
                 req.rounds += self.big_chunk
-                bad = not math.isfinite(st.time)
-                if bad or not (st.time < s["tlim"] and st.cycle < s["nlim"]) \
-                        or req.rounds > req.max_rounds:
-                    req.finished = self.now()
-                    req.info["cycles"] = int(st.cycle)
-                    if bad or not torch.isfinite(U).all():
-                        req.status, req.info["error"] = "error", "non-finite state"
-                    elif st.time < s["tlim"] and st.cycle < s["nlim"]:
-                        req.status, req.info["error"] = "error", "round budget exceeded"
-                    else:
-                        req.status = "done"
-                    req.result = (U, st.time, st.dt, st.cycle) if self.keep_results else None
-                    req.state = None
+                if self._grid_finish(req, U, st):
                     del self.grid_running[rid]
                     self.finished.append(req)
         self.grid_stream.synchronize()
@@ -241,22 +322,43 @@ class Server:
         for r in requests:
             self.submit(r)
         self.clock0 = time.perf_counter()
-        while self.queue or self.running or self.grid_running:
-            self._admit()
-            rounds = self.rounds_per_poll * (self.big_rounds if self.grid_running else 1)
-            if self.running:
-                self.ens.launch(rounds)
-            if self.grid_running:
-                self._step_grid()
-            if self.running:
-                self.ens.wait()
-                self._retire(rounds)
-            elif not self.grid_running and self.queue:
-                wait = min(r.arrival for r in self.queue) - self.now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-            if timeout is not None and self.now() > timeout:
-                break
+        self.stop.clear()
+        threaded = self.big_threads
+        if threaded:
+            # Large requests a timed-out call left running continue where they stopped.
+            for req in self.grid_running.values():
+                if req.rid not in self.workers:
+                    self._start_worker(req)
+        try:
+            while self.queue or self.running or self.grid_running:
+                self._admit()
+                if threaded:
+                    self._collect_grid()
+                busy = self.grid_running and not threaded
+                rounds = self.rounds_per_poll * (self.big_rounds if busy else 1)
+                if self.running:
+                    self.ens.launch(rounds)
+                if busy:
+                    self._step_grid()
+                if self.running:
+                    self.ens.wait()
+                    self._retire(rounds)
+                elif self.grid_running and threaded:
+                    time.sleep(0.0005)   # only large requests run: wait for their workers
+                elif not self.grid_running and self.queue:
+                    wait = min(r.arrival for r in self.queue) - self.now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                if timeout is not None and self.now() > timeout:
+                    break
+        finally:
+            # Workers stop between chunks; a request stopped early stays running with its
+            # state.
+            self.stop.set()
+            for th, _ in list(self.workers.values()):
+                th.join()
+            if threaded:
+                self._collect_grid()
         return self.finished
 
     @staticmethod

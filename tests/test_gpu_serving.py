@@ -59,18 +59,20 @@ def test_poisoned_request_is_evicted():
     assert all(out[k].status == "done" for k in (0, 2, 3))
 
 
-def test_large_requests_on_the_single_grid_path():
-    """Requests at or above big_zones cells run on the single-grid kernel beside the batch.
-    Served alone under the same threshold a request takes the same path and gives the same
-    result bit for bit; against the batched path the result agrees to FP32 round-off (the
-    two kernels are separate code that may fuse different products into adds, and forced
-    runs reduce their kick sums in another order)."""
+@pytest.mark.parametrize("threads", (True, False))
+def test_large_requests_on_the_single_grid_path(threads):
+    """Requests at or above big_zones cells run on the single-grid kernel beside the batch,
+    in worker threads or in chunks of the server loop.  Served alone under the same
+    threshold a request takes the same path and gives the same result bit for bit; against
+    the batched path the result agrees to FP32 round-off (the two kernels are separate code
+    that may fuse different products into adds, and forced runs reduce their kick sums in
+    another order)."""
     from jathenax_amd.serving import Request, Server
     reqs = _requests(n=6, seed=11)
     sizes = sorted(r.spec["U"].shape[0] * r.spec["U"].shape[1] for r in reqs)
     big = sizes[3]
     out = Server(pool_bytes=1 << 28, max_instances=8, big_zones=big, max_big=1,
-                 big_chunk=5).serve(reqs)
+                 big_chunk=5, big_threads=threads).serve(reqs)
     assert all(r.status == "done" for r in out), [(r.rid, r.status, r.info) for r in out]
     assert {r.path for r in out} == {"grid", "batch"}
     for r in out:
@@ -86,3 +88,24 @@ def test_large_requests_on_the_single_grid_path():
         a, b = batch.result[0].cpu().numpy(), r.result[0].cpu().numpy()
         assert batch.result[3] == r.result[3]
         assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b))
+
+
+def test_large_request_continues_after_a_timeout():
+    """A timed-out serve() stops a large request's worker between chunks; the request keeps
+    its state, the next serve() continues it, and the result equals an uninterrupted run
+    bit for bit."""
+    from jathenax_amd import workloads
+    from jathenax_amd.serving import Request, Server
+    # Kelvin-Helmholtz 128^2 to t = 1: hundreds of steps, so the worker stops long before
+    # the end.
+    spec = workloads.kh_spec(128, 128, 1.0, 0.5)
+    big = 128 * 128
+    srv = Server(pool_bytes=1 << 28, max_instances=2, big_zones=big, big_chunk=3)
+    first = srv.serve([Request(rid=0, spec=spec, arrival=0.0)], timeout=0.0)
+    assert first == [] and list(srv.grid_running) == [0]
+    (r,) = srv.serve([])
+    alone = Server(pool_bytes=1 << 28, max_instances=2, big_zones=big, big_chunk=3).serve(
+        [Request(rid=0, spec=spec, arrival=0.0)])[0]
+    assert r.status == alone.status == "done" and r.path == alone.path == "grid"
+    np.testing.assert_array_equal(r.result[0].cpu().numpy(), alone.result[0].cpu().numpy())
+    assert r.result[1:] == alone.result[1:]
