@@ -284,7 +284,8 @@ int jx_batch_poll(jx_batch* b, int* done, double* time, int* cycle, int cap);
  * polled (every launch snapshots the loop state; two are kept, a third drops the oldest;
  * jx_batch_wait, jx_batch_run and jx_batch_poll clear them).  Waits for that launch only,
  * so a caller that keeps the next launch queued never leaves the GPU idle while it reads.
- * No counterpart in the reference (batching is new). */
+ * A slot whose instance was added after that launch is reported running (0), at time NaN
+ * and cycle -1.  No counterpart in the reference (batching is new). */
 int jx_batch_poll_launched(jx_batch* b, int* done, double* time, int* cycle, int cap,
                            int* active_out);
 /* Copy an instance's state to U_out (may be NULL) and return (time, last dt, cycle). */

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -155,6 +156,7 @@ template <typename T> class Batch : public BatchBase {
     Slot& sl = slots_[s];
     sl = Slot{};
     sl.live = true;
+    sl.since = launches_;
     done_seen_[s] = 0;
     sl.off = (size_t)off;
     sl.bytes = need;
@@ -295,6 +297,7 @@ template <typename T> class Batch : public BatchBase {
       snap_count_ = 1;
     }
     const int k = snap_head_;
+    snap_seq_[k] = launches_++;
     if (hipMemcpyAsync(h_snap_[k], d_ctrl_, sizeof(BatchCtrl) * max_inst_,
                        hipMemcpyDeviceToHost, stream_) != hipSuccess ||
         hipMemcpyAsync(h_snap_meta_[k], d_meta_, sizeof(int) * 8, hipMemcpyDeviceToHost,
@@ -339,16 +342,26 @@ template <typename T> class Batch : public BatchBase {
     const int k = snap_tail_;
     if (hipEventSynchronize(snap_ev_[k]) != hipSuccess)
       return fail("poll_launched: synchronisation failed");
-    read_ctrl(h_snap_[k], done, time, cycle, cap);
+    read_ctrl(h_snap_[k], done, time, cycle, cap, snap_seq_[k]);
     if (active) *active = h_snap_meta_[k][2];
     snap_tail_ ^= 1;
     --snap_count_;
     return JX_OK;
   }
 
-  void read_ctrl(const BatchCtrl* h, int* done, double* time, int* cycle, int cap) {
+  // `seq`: the launch a snapshot was taken at (the current state: every launch so far).  A
+  // slot whose instance was added after that launch shows its previous occupant there: it
+  // is reported running, at time NaN and cycle -1, and its done flag is not recorded.
+  void read_ctrl(const BatchCtrl* h, int* done, double* time, int* cycle, int cap,
+                 long long seq = std::numeric_limits<long long>::max()) {
     for (int k = 0; k < std::min(cap, max_inst_); ++k) {
       const bool live = slots_[k].live;
+      if (live && seq < slots_[k].since) {
+        if (done) done[k] = 0;
+        if (time) time[k] = std::numeric_limits<double>::quiet_NaN();
+        if (cycle) cycle[k] = -1;
+        continue;
+      }
       if (live) done_seen_[k] = h[k].c.done ? 1 : 0;
       if (done) done[k] = live ? (h[k].c.done ? 1 : 0) : -1;
       if (time) time[k] = h[k].c.time;
@@ -422,6 +435,7 @@ template <typename T> class Batch : public BatchBase {
     bool live = false;
     size_t off = 0, bytes = 0;
     jx_run_params p{};
+    long long since = 0;   // index of the first launch the instance takes part in
   };
 
   static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
@@ -577,6 +591,8 @@ template <typename T> class Batch : public BatchBase {
   int* h_snap_meta_[2] = {};
   hipEvent_t snap_ev_[2] = {};
   int snap_head_ = 0, snap_tail_ = 0, snap_count_ = 0;
+  long long snap_seq_[2] = {};   // the launch index each snapshot was taken at
+  long long launches_ = 0;       // launches so far
   std::vector<BatchInst<T>> h_inst_;
   std::vector<Slot> slots_;
   std::vector<char> done_seen_;   // per slot: a poll has seen the instance done

@@ -64,6 +64,7 @@ class Request:
     priority: int = 0              # larger is more urgent
     checkpoint: object = None      # (U on device, run.LoopState) while pre-empted
     preemptions: int = 0
+    first_launch: int = 0          # the first batch launch it takes part in (batch path)
     info: dict = field(default_factory=dict)
     path: str = "batch"            # "batch" or "grid" (single-grid kernel, large requests)
     state: object = None           # grid path: (U on device, run.LoopState) between chunks
@@ -108,6 +109,10 @@ class Server:
         self.workers = {}
         self.stop = threading.Event()
         self.finished = []
+        # Batch launches issued so far; an admitted request takes part in launches from
+        # index first_launch on, so a loop-state snapshot of an earlier launch (which may show
+        # the slot's previous occupant) is not applied to it.
+        self.launches = 0
         self.clock0 = None
 
     def now(self):
@@ -214,12 +219,17 @@ class Server:
             if math.isnan(req.admitted):
                 req.admitted = self.now()
             req.status = "running"
+            req.first_launch = self.launches
             self.running[req.iid] = req
 
-    def _retire(self, rounds):
-        done, times, cycles = self.ens.poll()
+    def _retire(self, rounds, launch, snapshot=None):
+        """Retire the requests that the loop state as of the end of batch launch `launch`
+        (`snapshot`: done, time, cycle; None: poll the current state) shows finished."""
+        done, times, cycles = snapshot if snapshot is not None else self.ens.poll()
         for iid in list(self.running):
             req = self.running[iid]
+            if req.first_launch > launch:
+                continue   # admitted after that launch: the snapshot predates it
             req.rounds += rounds
             bad = not math.isfinite(times[iid])
             if done[iid] == 1 or bad or req.rounds > req.max_rounds:
@@ -329,8 +339,13 @@ class Server:
             for req in self.grid_running.values():
                 if req.rid not in self.workers:
                     self._start_worker(req)
+        # Batch launches not yet read: (index, rounds).  While requests run, the next launch
+        # is queued before the loop state of the previous one is read (poll_launched waits
+        # for that launch only), so the batch's rounds run back to back; adding or removing
+        # an instance waits for the launches in flight.
+        inflight = deque()
         try:
-            while self.queue or self.running or self.grid_running:
+            while self.queue or self.running or self.grid_running or inflight:
                 self._admit()
                 if threaded:
                     self._collect_grid()
@@ -338,11 +353,20 @@ class Server:
                 rounds = self.rounds_per_poll * (self.big_rounds if busy else 1)
                 if self.running:
                     self.ens.launch(rounds)
+                    inflight.append((self.launches, rounds))
+                    self.launches += 1
                 if busy:
                     self._step_grid()
-                if self.running:
+                if len(inflight) > 1:
+                    done, times, cycles, _ = self.ens.poll_launched()
+                    launch, n = inflight.popleft()
+                    self._retire(n, launch, (done, times, cycles))
+                elif inflight and not self.running:
                     self.ens.wait()
-                    self._retire(rounds)
+                    launch, n = inflight.popleft()
+                    self._retire(n, launch)
+                elif inflight:
+                    pass   # the first launch of a run of rounds: read it next iteration
                 elif self.grid_running and threaded:
                     time.sleep(0.0005)   # only large requests run: wait for their workers
                 elif not self.grid_running and self.queue:
@@ -352,6 +376,11 @@ class Server:
                 if timeout is not None and self.now() > timeout:
                     break
         finally:
+            if inflight:   # a timed-out call: settle the launches in flight
+                self.ens.wait()
+                while inflight:
+                    launch, n = inflight.popleft()
+                    self._retire(n, launch)
             # Workers stop between chunks; a request stopped early stays running with its
             # state.
             self.stop.set()

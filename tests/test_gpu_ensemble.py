@@ -253,3 +253,30 @@ def test_pipelined_polling(jx):
         ra, rb = a.result(i), b.result(j)
         np.testing.assert_array_equal(_np(ra[0]), _np(rb[0]))
         assert ra[1:] == rb[1:]
+
+
+def test_polled_snapshot_of_a_reused_slot(jx):
+    """A launch's snapshot taken before an instance was added to a slot shows the slot's
+    previous occupant: poll_launched reports such a slot running, at time NaN and cycle -1,
+    and the first snapshot taken after the add reports the new instance."""
+    from jathenax_amd.ensemble import Ensemble
+    s = _specs()[0]
+    ens = Ensemble(dtype=torch.float32, max_instances=1, pool_bytes=1 << 26)
+
+    def add():
+        return ens.add(torch.tensor(s["U"], dtype=torch.float32, device="cuda"), s["dx"],
+                       s["dy"], 1.0, 1e-10, s["integ"], 0.4, 0.05, 10 ** 6, **s["kw"])
+
+    a = add()
+    ens.launch(6)
+    ens.launch(6)
+    ens.remove(a)
+    b = add()
+    assert b == a
+    for _ in range(2):   # the two snapshots taken before the add
+        done, t, c, _ = ens.poll_launched()
+        assert done[b] == 0 and np.isnan(t[b]) and c[b] == -1
+    ens.launch(6)
+    done, t, c, _ = ens.poll_launched()
+    assert done[b] in (0, 1) and np.isfinite(t[b]) and c[b] >= 0
+    ens.wait()
